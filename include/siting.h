@@ -1,0 +1,200 @@
+/*
+ * siting.h -- C ABI of the MI355X-native siting library (libsiting_amd.so).
+ *
+ * Four stages, as in the reference (siting.cpp / siting.cu):
+ *   VIX       approximate visibility index per terrain point
+ *   FINDMAX   top vix points per block become tentative observers
+ *   VIEWSHED  one packed MSB-first bitset per tentative observer
+ *   SITE      greedy max-coverage selection over the bitsets
+ *
+ * Plain pointers and sizes only.  Results are bit-exact against the
+ * reference's sequential build (siting.cpp -DSEQ): vix bytes, obs indices,
+ * every shed word, the selected mask and the coverage.
+ *
+ * Errors are returned as siting_status codes instead of the reference's
+ * die()/exit(1) (siting.cpp:22-25); siting_status_str() gives the message.
+ *
+ * Data layouts (identical to the reference's host arrays):
+ *   elevs     uint16[nrows*nrows], elevs[x*nrows+y], x = row     (cpp:709)
+ *   vix       uint8 [nrows*nrows]                                (cpp:710)
+ *   obs       int32 [nwanted], obs[(bx*nblocks+by)*nwpb+k]=x*nrows+y (cpp:711)
+ *   sheds     uint64[nwanted*nr*nwpr], nr = 2*roi+1, nwpr = ceil(nr/64);
+ *             bit (row,col) of shed k is word k*nr*nwpr + row*nwpr + col/64,
+ *             bit 63 - col%64; row r is terrain row x_obs - roi + r  (cpp:247-250)
+ *   selected  uint8 [nwanted], 1 for a selected observer          (cpp:713)
+ */
+#ifndef SITING_H
+#define SITING_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  SITING_OK = 0,
+  SITING_EINVAL = 1,      /* bad pointer / size / parameter outside what the
+                             library can run */
+  SITING_ERANGE = 2,      /* parameter outside the reference CLI's ranges
+                             (siting.cpp:680-686); siting_check_cli only */
+  SITING_ESMALLBLOCK = 3, /* "The last block is too small for
+                             nwantedperblock." (siting.cpp:694-695) */
+  SITING_ENOMEM = 4,      /* device or host allocation failed */
+  SITING_EDEVICE = 5,     /* HIP runtime error or no usable GPU */
+  SITING_EIO = 6,         /* "Input failed" (siting.cpp:726-729) / write error */
+  SITING_EBUSY = 7,       /* server: no admission possible right now */
+  SITING_ENOTFOUND = 8    /* server: unknown job id */
+} siting_status;
+
+const char *siting_status_str(siting_status s);
+
+/* Run parameters: the reference's argv (siting.cpp:668-674).  The CLI sets
+ * oht = tht = height (cpp:670); the library keeps them separate. */
+typedef struct {
+  int nrows;
+  int roi;
+  int oht;
+  int tht;
+  int ntests;   /* "targets" */
+  int bw;       /* requested block width (blocksize0) */
+  int nwanted0; /* requested tentative observers */
+  int intervis; /* 0 or 1 */
+} siting_params;
+
+/* Derived sizes (siting.cpp:689-702). */
+typedef struct {
+  float blocksize; /* (float)n / (int)((float)n/bw + 0.5f)        cpp:689 */
+  int nblocks;     /* (int)(n/blocksize + 0.5f)                   cpp:690 */
+  int nwpb;        /* (int)((float)nwanted0/nblocks^2 + 0.99999f) cpp:691 */
+  int nwanted;     /* nwpb * nblocks^2                            cpp:692 */
+  int lastsize;    /* n - (int)(blocksize*(nblocks-1))            cpp:693 */
+  int nr;          /* 2*roi + 1                                   cpp:701 */
+  int nwpr;        /* (nr + 63) / 64                              cpp:702 */
+  int cumnwpr;     /* (n + 63) / 64                               cpp:505 */
+  int64_t shed_words; /* nr * nwpr                                          */
+} siting_derived;
+
+/* Per-job counters and device stage times. */
+typedef struct {
+  int nused;        /* selected observers (" nusedsheds:N", cpp:617) */
+  int niter;        /* SITE loop iterations, the final one included */
+  int64_t cumarea;  /* covered terrain points */
+  int64_t sum_u;    /* sum over iterations of |update list| (cpp:540-544) */
+  double coverage;  /* 100.0*cumarea/nrows^2 (" coverage:C", cpp:617) */
+  float ms_vix, ms_findmax, ms_viewshed, ms_site; /* HIP event times */
+} siting_stats;
+
+/* ---- parameters ------------------------------------------------------- */
+
+/* siting.cpp:689-702 without the CLI's range checks (so bw < 10 runs, as a
+ * library call of the reference's stage functions does).  Returns
+ * SITING_ESMALLBLOCK where the reference dies with "last block too small",
+ * SITING_EINVAL for nonsensical values (nrows, roi, ntests < 1, bw < 1,
+ * intervis not 0/1, nrows/bw rounding to zero blocks, sizes whose arrays
+ * overflow 32-bit indexing). */
+siting_status siting_derive(const siting_params *p, siting_derived *d);
+
+/* The reference CLI's range checks (siting.cpp:680-686), in its order.
+ * Returns SITING_OK or SITING_ERANGE with *which = 1..7 (nrows, roi, tht,
+ * ntests, bw, nwanted0, intervis) and *msg = the reference's die() text. */
+siting_status siting_check_cli(const siting_params *p, int *which,
+                               const char **msg);
+
+/* ---- stage functions on host arrays (drop-in for the reference's) -----
+ * Each uploads its inputs, runs the HIP kernels on the library's default
+ * context (device 0) and downloads its outputs, like cusite's host
+ * wrappers (siting.cu:133,226,336,596) but with status returns. */
+
+/* calc_vix, siting.cpp:192 / siting.cu:133 */
+siting_status siting_calc_vix(int nrows, const uint16_t *elevs, int roi,
+                              int oht, int tht, int ntests, uint8_t *vix);
+
+/* find_max, siting.cpp:235 / siting.cu:226.  vix is mutated as in the
+ * sequential reference (picked cells set to 0, cpp:231). */
+siting_status siting_find_max(int nrows, uint8_t *vix, float blocksize,
+                              int nwanted, int nblocks, int nwpb,
+                              int32_t *obs);
+
+/* calc_sheds, siting.cpp:349 / siting.cu:336 */
+siting_status siting_calc_sheds(int nrows, const uint16_t *elevs, int roi,
+                                int oht, int tht, int nsheds,
+                                const int32_t *obs, uint64_t *sheds);
+
+/* site_it, siting.cpp:501 / siting.cu:596, without the stdout print (the
+ * counters come back in *stats, which may be NULL). */
+siting_status siting_site_it(int nrows, int roi, int intervis, int nsheds,
+                             const int32_t *obs, const uint64_t *sheds,
+                             uint8_t *selected, siting_stats *stats);
+
+/* ---- device-resident jobs --------------------------------------------
+ * A job owns its device buffers (DEM, vix, obs, sheds, SITE state), so the
+ * four stages run back to back in HBM with no host round trip.  All work of
+ * a job goes to one HIP stream owned by the job. */
+
+typedef struct siting_job siting_job;
+
+/* Device memory a job needs (bytes), for admission control. */
+siting_status siting_job_bytes(const siting_params *p, size_t *bytes);
+
+siting_status siting_job_create(int device, const siting_params *p,
+                                siting_job **out);
+void siting_job_destroy(siting_job *job);
+
+/* Copy a host DEM (nrows^2 uint16) into the job's device buffer. */
+siting_status siting_job_upload_dem(siting_job *job, const uint16_t *elevs);
+
+/* Enqueue the stages (asynchronous on the job's stream). */
+siting_status siting_job_vix(siting_job *job);
+siting_status siting_job_findmax(siting_job *job);
+siting_status siting_job_viewshed(siting_job *job);
+siting_status siting_job_site(siting_job *job);
+/* All four, in order. */
+siting_status siting_job_run(siting_job *job);
+
+/* Wait for the job's stream. */
+siting_status siting_job_sync(siting_job *job);
+
+/* Buffers that can be read back (after siting_job_sync). */
+typedef enum {
+  SITING_BUF_ELEVS = 0,
+  SITING_BUF_VIX = 1,
+  SITING_BUF_OBS = 2,
+  SITING_BUF_SHEDS = 3,
+  SITING_BUF_SELECTED = 4,
+  SITING_BUF_ORDER = 5 /* int32[nused]: selection order */
+} siting_buffer;
+
+siting_status siting_job_download(siting_job *job, siting_buffer which,
+                                  void *host, size_t bytes);
+/* Device address of a job buffer (for callers that keep data in HBM). */
+siting_status siting_job_device_ptr(siting_job *job, siting_buffer which,
+                                    void **dptr, size_t *bytes);
+/* Counters + per-stage device times of the last run (after sync). */
+siting_status siting_job_stats(siting_job *job, siting_stats *st);
+siting_status siting_job_derived(siting_job *job, siting_derived *d);
+
+/* ---- terrain input (raw uint16 format, siting.cpp:726-730) ------------ */
+
+/* Exactly nrows*nrows native-endian uint16 values; SITING_EIO if short. */
+siting_status siting_dem_read(const char *path, int nrows, uint16_t *out);
+siting_status siting_dem_write(const char *path, int nrows,
+                               const uint16_t *dem);
+/* Seeded diamond-square terrain with roughness rough_h, rescaled linearly to
+ * [hmin, hmax], plus nhills Gaussian hills (peak, sigma in cells). */
+siting_status siting_dem_generate(int nrows, double rough_h, int hmin,
+                                  int hmax, int nhills, double hill_peak,
+                                  double hill_sigma, uint64_t seed,
+                                  uint16_t *out);
+
+/* ---- library info ----------------------------------------------------- */
+
+/* Number of visible GPUs (0 when none); never fails. */
+int siting_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SITING_H */

@@ -1,0 +1,559 @@
+// api.hip -- the C ABI (include/siting.h): parameter derivation, device
+// resident jobs, and the reference-shaped stage functions on host arrays.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+
+namespace siting {
+
+static thread_local char g_last_error[256];
+
+siting_status hip_fail(hipError_t e, const char *what, const char *file, int line) {
+  std::snprintf(g_last_error, sizeof g_last_error, "%s failed: %s (%s:%d)", what,
+                hipGetErrorString(e), file, line);
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return SITING_ENOMEM;
+  return SITING_EDEVICE;
+}
+
+// siting.cpp:689-702, in float32 exactly as the reference evaluates it.
+siting_status derive(const siting_params *p, siting_derived *d) {
+  if (!p || !d) return SITING_EINVAL;
+  const int n = p->nrows;
+  if (n <= 0 || n > 32767 || p->roi < 1 || p->roi > 16383 || p->ntests < 1 ||
+      p->ntests > 4000 || p->bw < 1 || p->nwanted0 < 1 || p->oht < 0 ||
+      p->tht < 0 || p->oht > 1000000 || p->tht > 1000000 ||
+      (p->intervis != 0 && p->intervis != 1))
+    return SITING_EINVAL;
+  const float fn = (float)n;
+  const int nb_req = (int)(fn / (float)p->bw + 0.5f);
+  if (nb_req <= 0) return SITING_EINVAL;
+  const float bs = fn / (float)nb_req;
+  const int nb = (int)(fn / bs + 0.5f);
+  if (nb <= 0) return SITING_EINVAL;
+  const long long nb2 = (long long)nb * nb;
+  const int nwpb = (int)((float)p->nwanted0 / (float)nb2 + 0.99999f);
+  const long long nwanted = (long long)nwpb * nb2;
+  const int lastsize = n - (int)(bs * (float)(nb - 1));
+  d->blocksize = bs;
+  d->nblocks = nb;
+  d->nwpb = nwpb;
+  d->nwanted = (int)nwanted;
+  d->lastsize = lastsize;
+  d->nr = 2 * p->roi + 1;
+  d->nwpr = (d->nr + 63) / 64;
+  d->cumnwpr = (n + 63) / 64;
+  d->shed_words = (int64_t)d->nr * d->nwpr;
+  if ((long long)lastsize * lastsize < nwpb) return SITING_ESMALLBLOCK;
+  if (nwanted > 0x7fffffffLL || nwpb < 1 || lastsize < 1) return SITING_EINVAL;
+  // every block must hold at least one cell (blocksize >= 1 guarantees it)
+  if (!(bs >= 1.0f)) return SITING_EINVAL;
+  return SITING_OK;
+}
+
+}  // namespace siting
+
+using namespace siting;
+
+// ---------------------------------------------------------------------------
+// job
+// ---------------------------------------------------------------------------
+struct siting_job {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Geometry g{};
+  void *base = nullptr;
+  size_t bytes = 0;
+  uint16_t *elevs = nullptr;
+  uint8_t *vix = nullptr;
+  int32_t *obs = nullptr;
+  uint64_t *sheds = nullptr;
+  SiteBuffers site{};
+  float *recip = nullptr;
+  uint32_t *jump = nullptr;
+  hipEvent_t ev_start[4] = {};
+  hipEvent_t ev_end[4] = {};
+  bool ran[4] = {};
+};
+
+namespace {
+
+struct Layout {
+  size_t elevs, vix, obs, sheds, area, used, cum, rowmax, order, state, recip,
+      jump, scratch, total;
+};
+
+size_t round_up(size_t v) { return (v + 255) / 256 * 256; }
+
+Layout layout_for(const siting_params &p, const siting_derived &d) {
+  Layout L{};
+  const size_t n2 = (size_t)p.nrows * p.nrows;
+  const size_t nw = (size_t)d.nwanted;
+  size_t off = 0;
+  auto put = [&](size_t &field, size_t bytes) {
+    field = off;
+    off += round_up(bytes ? bytes : 1);
+  };
+  put(L.elevs, n2 * 2);
+  put(L.vix, n2);
+  put(L.obs, nw * 4);
+  put(L.sheds, nw * (size_t)d.shed_words * 8);
+  put(L.area, nw * 4);
+  put(L.used, nw);
+  put(L.cum, (size_t)p.nrows * d.cumnwpr * 8);
+  put(L.rowmax, 2 * (size_t)site_key_rows(p.nrows, p.roi) * 8);
+  put(L.order, nw * 4);
+  put(L.state, sizeof(SiteState));
+  put(L.recip, ((size_t)p.roi + 1) * 4);
+  put(L.jump, 4 * (size_t)p.ntests * 4);
+  put(L.scratch, site_scratch_bytes(p.nrows, p.roi, d.nwanted));
+  L.total = off;
+  return L;
+}
+
+siting_status record(hipEvent_t e, hipStream_t s) {
+  SITING_HIP(hipEventRecord(e, s));
+  return SITING_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *siting_status_str(siting_status s) {
+  switch (s) {
+    case SITING_OK: return "ok";
+    case SITING_EINVAL: return "invalid argument";
+    case SITING_ERANGE: return "parameter out of range";
+    case SITING_ESMALLBLOCK: return "The last block is too small for nwantedperblock.";
+    case SITING_ENOMEM: return "Memory exhausted. Program terminates.";
+    case SITING_EDEVICE: return g_last_error[0] ? g_last_error : "HIP device error";
+    case SITING_EIO: return "Input failed";
+    case SITING_EBUSY: return "busy";
+    case SITING_ENOTFOUND: return "not found";
+  }
+  return "unknown status";
+}
+
+siting_status siting_derive(const siting_params *p, siting_derived *d) {
+  return derive(p, d);
+}
+
+siting_status siting_check_cli(const siting_params *p, int *which, const char **msg) {
+  static const char *msgs[] = {
+      "", "Unreasonable value for nrows", "Unreasonable value for roi.",
+      "Unreasonable value for tht.", "Unreasonable value for ntests.",
+      "Unreasonable value for blocksize0.", "Unreasonable value for nwanted0.",
+      "Unreasonable value for intervis."};
+  int w = 0;  // siting.cpp:680-686, in order
+  if (!p) return SITING_EINVAL;
+  if (p->nrows <= 0 || p->nrows > 20000) w = 1;
+  else if (p->roi < 1 || p->roi > 10000) w = 2;
+  else if (p->tht < 0 || p->tht > 1000000) w = 3;
+  else if (p->ntests < 1 || p->ntests > 1000) w = 4;
+  else if (p->bw < 10 || p->bw > 2000) w = 5;
+  else if (p->nwanted0 < 100 || p->nwanted0 > 2000000) w = 6;
+  else if (p->intervis != 0 && p->intervis != 1) w = 7;
+  if (which) *which = w;
+  if (msg) *msg = msgs[w];
+  return w ? SITING_ERANGE : SITING_OK;
+}
+
+int siting_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+siting_status siting_job_bytes(const siting_params *p, size_t *bytes) {
+  siting_derived d;
+  siting_status st = derive(p, &d);
+  if (st != SITING_OK) return st;
+  if (bytes) *bytes = layout_for(*p, d).total;
+  return SITING_OK;
+}
+
+siting_status siting_job_create(int device, const siting_params *p, siting_job **out) {
+  if (!out) return SITING_EINVAL;
+  *out = nullptr;
+  siting_derived d;
+  siting_status st = derive(p, &d);
+  if (st != SITING_OK) return st;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+    std::snprintf(g_last_error, sizeof g_last_error, "no HIP device %d", device);
+    return SITING_EDEVICE;
+  }
+  SITING_HIP(hipSetDevice(device));
+  siting_job *j = new siting_job();
+  j->device = device;
+  j->g.p = *p;
+  j->g.d = d;
+  Layout L = layout_for(*p, d);
+  j->bytes = L.total;
+  auto fail = [&](siting_status s) {
+    siting_job_destroy(j);
+    return s;
+  };
+  hipError_t e = hipMalloc(&j->base, L.total);
+  if (e != hipSuccess) {
+    j->base = nullptr;
+    return fail(hip_fail(e, "hipMalloc(job)", __FILE__, __LINE__));
+  }
+  e = hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return fail(hip_fail(e, "hipStreamCreate", __FILE__, __LINE__));
+  for (int i = 0; i < 4; i++) {
+    if ((e = hipEventCreate(&j->ev_start[i])) != hipSuccess ||
+        (e = hipEventCreate(&j->ev_end[i])) != hipSuccess)
+      return fail(hip_fail(e, "hipEventCreate", __FILE__, __LINE__));
+  }
+  uint8_t *b = (uint8_t *)j->base;
+  j->elevs = (uint16_t *)(b + L.elevs);
+  j->vix = b + L.vix;
+  j->obs = (int32_t *)(b + L.obs);
+  j->sheds = (uint64_t *)(b + L.sheds);
+  j->site.area = (int32_t *)(b + L.area);
+  j->site.used = b + L.used;
+  j->site.cum = (uint64_t *)(b + L.cum);
+  j->site.rowmax = (uint64_t *)(b + L.rowmax);
+  j->site.order = (int32_t *)(b + L.order);
+  j->site.state = (SiteState *)(b + L.state);
+  j->site.scratch = b + L.scratch;
+  j->recip = (float *)(b + L.recip);
+  j->jump = (uint32_t *)(b + L.jump);
+  std::vector<float> rec((size_t)p->roi + 1);
+  std::vector<uint32_t> jmp(4 * (size_t)p->ntests);
+  fill_recip_table(p->roi, rec.data());
+  fill_jump_table(p->ntests, jmp.data());
+  e = hipMemcpy(j->recip, rec.data(), rec.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(j->jump, jmp.data(), jmp.size() * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail(hip_fail(e, "hipMemcpy(tables)", __FILE__, __LINE__));
+  *out = j;
+  return SITING_OK;
+}
+
+void siting_job_destroy(siting_job *j) {
+  if (!j) return;
+  (void)hipSetDevice(j->device);
+  if (j->stream) (void)hipStreamSynchronize(j->stream);
+  for (int i = 0; i < 4; i++) {
+    if (j->ev_start[i]) (void)hipEventDestroy(j->ev_start[i]);
+    if (j->ev_end[i]) (void)hipEventDestroy(j->ev_end[i]);
+  }
+  if (j->stream) (void)hipStreamDestroy(j->stream);
+  if (j->base) (void)hipFree(j->base);
+  delete j;
+}
+
+siting_status siting_job_upload_dem(siting_job *j, const uint16_t *elevs) {
+  if (!j || !elevs) return SITING_EINVAL;
+  SITING_HIP(hipSetDevice(j->device));
+  const size_t bytes = (size_t)j->g.p.nrows * j->g.p.nrows * 2;
+  SITING_HIP(hipMemcpyAsync(j->elevs, elevs, bytes, hipMemcpyHostToDevice, j->stream));
+  SITING_HIP(hipStreamSynchronize(j->stream));
+  return SITING_OK;
+}
+
+#define JOB_STAGE(idx, body)                                         \
+  do {                                                               \
+    if (!j) return SITING_EINVAL;                                    \
+    SITING_HIP(hipSetDevice(j->device));                             \
+    siting_status st_ = record(j->ev_start[idx], j->stream);         \
+    if (st_ != SITING_OK) return st_;                                \
+    st_ = (body);                                                    \
+    if (st_ != SITING_OK) return st_;                                \
+    st_ = record(j->ev_end[idx], j->stream);                         \
+    if (st_ != SITING_OK) return st_;                                \
+    j->ran[idx] = true;                                              \
+    return SITING_OK;                                                \
+  } while (0)
+
+siting_status siting_job_vix(siting_job *j) {
+  JOB_STAGE(0, launch_vix(j->g, j->elevs, j->vix, VixTables{j->recip, j->jump}, j->stream));
+}
+siting_status siting_job_findmax(siting_job *j) {
+  JOB_STAGE(1, launch_findmax(j->g, j->vix, j->obs, j->stream));
+}
+siting_status siting_job_viewshed(siting_job *j) {
+  JOB_STAGE(2, launch_viewshed(j->g, j->elevs, j->obs, j->sheds, j->stream));
+}
+siting_status siting_job_site(siting_job *j) {
+  JOB_STAGE(3, launch_site(j->g, j->obs, j->sheds, j->site, j->device, j->stream));
+}
+
+siting_status siting_job_run(siting_job *j) {
+  siting_status s;
+  if ((s = siting_job_vix(j)) != SITING_OK) return s;
+  if ((s = siting_job_findmax(j)) != SITING_OK) return s;
+  if ((s = siting_job_viewshed(j)) != SITING_OK) return s;
+  return siting_job_site(j);
+}
+
+siting_status siting_job_sync(siting_job *j) {
+  if (!j) return SITING_EINVAL;
+  SITING_HIP(hipSetDevice(j->device));
+  SITING_HIP(hipStreamSynchronize(j->stream));
+  return SITING_OK;
+}
+
+siting_status siting_job_device_ptr(siting_job *j, siting_buffer which, void **dptr,
+                                    size_t *bytes) {
+  if (!j) return SITING_EINVAL;
+  const size_t n2 = (size_t)j->g.p.nrows * j->g.p.nrows;
+  const size_t nw = (size_t)j->g.d.nwanted;
+  void *p = nullptr;
+  size_t b = 0;
+  switch (which) {
+    case SITING_BUF_ELEVS: p = j->elevs; b = n2 * 2; break;
+    case SITING_BUF_VIX: p = j->vix; b = n2; break;
+    case SITING_BUF_OBS: p = j->obs; b = nw * 4; break;
+    case SITING_BUF_SHEDS: p = j->sheds; b = nw * (size_t)j->g.d.shed_words * 8; break;
+    case SITING_BUF_SELECTED: p = j->site.used; b = nw; break;
+    case SITING_BUF_ORDER: p = j->site.order; b = nw * 4; break;
+    default: return SITING_EINVAL;
+  }
+  if (dptr) *dptr = p;
+  if (bytes) *bytes = b;
+  return SITING_OK;
+}
+
+siting_status siting_job_download(siting_job *j, siting_buffer which, void *host,
+                                  size_t bytes) {
+  void *p = nullptr;
+  size_t b = 0;
+  siting_status st = siting_job_device_ptr(j, which, &p, &b);
+  if (st != SITING_OK) return st;
+  if (!host || bytes > b) return SITING_EINVAL;
+  SITING_HIP(hipSetDevice(j->device));
+  SITING_HIP(hipMemcpyAsync(host, p, bytes, hipMemcpyDeviceToHost, j->stream));
+  SITING_HIP(hipStreamSynchronize(j->stream));
+  return SITING_OK;
+}
+
+siting_status siting_job_stats(siting_job *j, siting_stats *st) {
+  if (!j || !st) return SITING_EINVAL;
+  SITING_HIP(hipSetDevice(j->device));
+  SITING_HIP(hipStreamSynchronize(j->stream));
+  std::memset(st, 0, sizeof *st);
+  float *ms[4] = {&st->ms_vix, &st->ms_findmax, &st->ms_viewshed, &st->ms_site};
+  for (int i = 0; i < 4; i++)
+    if (j->ran[i]) SITING_HIP(hipEventElapsedTime(ms[i], j->ev_start[i], j->ev_end[i]));
+  if (j->ran[3]) {
+    SiteState s;
+    SITING_HIP(hipMemcpy(&s, j->site.state, sizeof s, hipMemcpyDeviceToHost));
+    st->nused = s.nused;
+    st->niter = s.niter;
+    st->cumarea = (int64_t)s.cumarea;
+    st->sum_u = (int64_t)s.sum_u;
+    const double n2 = (double)j->g.p.nrows * (double)j->g.p.nrows;
+    st->coverage = 100.0 * (double)s.cumarea / n2;
+  }
+  return SITING_OK;
+}
+
+siting_status siting_job_derived(siting_job *j, siting_derived *d) {
+  if (!j || !d) return SITING_EINVAL;
+  *d = j->g.d;
+  return SITING_OK;
+}
+
+// ---------------------------------------------------------------------------
+// stage functions on host arrays (the reference's signatures)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+struct StageStream {
+  hipStream_t s = nullptr;
+  ~StageStream() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
+siting_status stage_device(hipStream_t &s, StageStream &holder) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    std::snprintf(g_last_error, sizeof g_last_error, "no HIP device");
+    return SITING_EDEVICE;
+  }
+  SITING_HIP(hipSetDevice(0));
+  SITING_HIP(hipStreamCreateWithFlags(&holder.s, hipStreamNonBlocking));
+  s = holder.s;
+  return SITING_OK;
+}
+
+siting_status dalloc(DevBuf &b, size_t bytes) {
+  SITING_HIP(hipMalloc(&b.p, bytes ? bytes : 1));
+  return SITING_OK;
+}
+
+}  // namespace
+
+#define CHECK(x)                             \
+  do {                                       \
+    siting_status s_ = (x);                  \
+    if (s_ != SITING_OK) return s_;          \
+  } while (0)
+
+siting_status siting_calc_vix(int nrows, const uint16_t *elevs, int roi, int oht,
+                              int tht, int ntests, uint8_t *vix) {
+  if (!elevs || !vix || nrows <= 0 || nrows > 32767 || roi < 1 || roi > 16383 ||
+      ntests < 1 || ntests > 4000)
+    return SITING_EINVAL;
+  hipStream_t s;
+  StageStream hs;
+  CHECK(stage_device(s, hs));
+  Geometry g{};
+  g.p.nrows = nrows;
+  g.p.roi = roi;
+  g.p.oht = oht;
+  g.p.tht = tht;
+  g.p.ntests = ntests;
+  const size_t n2 = (size_t)nrows * nrows;
+  DevBuf de, dv, dr, dj;
+  CHECK(dalloc(de, n2 * 2));
+  CHECK(dalloc(dv, n2));
+  CHECK(dalloc(dr, ((size_t)roi + 1) * 4));
+  CHECK(dalloc(dj, 16 * (size_t)ntests));
+  std::vector<float> rec((size_t)roi + 1);
+  std::vector<uint32_t> jmp(4 * (size_t)ntests);
+  fill_recip_table(roi, rec.data());
+  fill_jump_table(ntests, jmp.data());
+  SITING_HIP(hipMemcpyAsync(de.p, elevs, n2 * 2, hipMemcpyHostToDevice, s));
+  SITING_HIP(hipMemcpyAsync(dr.p, rec.data(), rec.size() * 4, hipMemcpyHostToDevice, s));
+  SITING_HIP(hipMemcpyAsync(dj.p, jmp.data(), jmp.size() * 4, hipMemcpyHostToDevice, s));
+  CHECK(launch_vix(g, (const uint16_t *)de.p, (uint8_t *)dv.p,
+                   VixTables{(const float *)dr.p, (const uint32_t *)dj.p}, s));
+  SITING_HIP(hipMemcpyAsync(vix, dv.p, n2, hipMemcpyDeviceToHost, s));
+  SITING_HIP(hipStreamSynchronize(s));
+  return SITING_OK;
+}
+
+siting_status siting_find_max(int nrows, uint8_t *vix, float blocksize, int nwanted,
+                              int nblocks, int nwpb, int32_t *obs) {
+  (void)nwanted;  // unused by the reference too (siting.cpp:235)
+  if (!vix || !obs || nrows <= 0 || nrows > 32767 || nblocks < 1 || nwpb < 0 ||
+      !(blocksize >= 1.0f))
+    return SITING_EINVAL;
+  hipStream_t s;
+  StageStream hs;
+  CHECK(stage_device(s, hs));
+  Geometry g{};
+  g.p.nrows = nrows;
+  g.d.blocksize = blocksize;
+  g.d.nblocks = nblocks;
+  g.d.nwpb = nwpb;
+  const size_t n2 = (size_t)nrows * nrows;
+  const size_t nobs = (size_t)nblocks * nblocks * nwpb;
+  DevBuf dv, dobs;
+  CHECK(dalloc(dv, n2));
+  CHECK(dalloc(dobs, nobs * 4));
+  SITING_HIP(hipMemcpyAsync(dv.p, vix, n2, hipMemcpyHostToDevice, s));
+  CHECK(launch_findmax(g, (uint8_t *)dv.p, (int32_t *)dobs.p, s));
+  SITING_HIP(hipMemcpyAsync(obs, dobs.p, nobs * 4, hipMemcpyDeviceToHost, s));
+  SITING_HIP(hipMemcpyAsync(vix, dv.p, n2, hipMemcpyDeviceToHost, s));
+  SITING_HIP(hipStreamSynchronize(s));
+  return SITING_OK;
+}
+
+siting_status siting_calc_sheds(int nrows, const uint16_t *elevs, int roi, int oht,
+                                int tht, int nsheds, const int32_t *obs,
+                                uint64_t *sheds) {
+  if (!elevs || !obs || !sheds || nrows <= 0 || nrows > 32767 || roi < 1 ||
+      roi > 16383 || nsheds < 0)
+    return SITING_EINVAL;
+  const size_t n2 = (size_t)nrows * nrows;
+  for (int k = 0; k < nsheds; k++)
+    if (obs[k] < 0 || (size_t)obs[k] >= n2) return SITING_EINVAL;
+  if (nsheds == 0) return SITING_OK;
+  hipStream_t s;
+  StageStream hs;
+  CHECK(stage_device(s, hs));
+  Geometry g{};
+  g.p.nrows = nrows;
+  g.p.roi = roi;
+  g.p.oht = oht;
+  g.p.tht = tht;
+  g.d.nwanted = nsheds;
+  g.d.nr = 2 * roi + 1;
+  g.d.nwpr = (g.d.nr + 63) / 64;
+  const size_t sbytes = (size_t)nsheds * g.d.nr * g.d.nwpr * 8;
+  DevBuf de, dobs, dsh;
+  CHECK(dalloc(de, n2 * 2));
+  CHECK(dalloc(dobs, (size_t)nsheds * 4));
+  CHECK(dalloc(dsh, sbytes));
+  SITING_HIP(hipMemcpyAsync(de.p, elevs, n2 * 2, hipMemcpyHostToDevice, s));
+  SITING_HIP(hipMemcpyAsync(dobs.p, obs, (size_t)nsheds * 4, hipMemcpyHostToDevice, s));
+  CHECK(launch_viewshed(g, (const uint16_t *)de.p, (const int32_t *)dobs.p,
+                        (uint64_t *)dsh.p, s));
+  SITING_HIP(hipMemcpyAsync(sheds, dsh.p, sbytes, hipMemcpyDeviceToHost, s));
+  SITING_HIP(hipStreamSynchronize(s));
+  return SITING_OK;
+}
+
+siting_status siting_site_it(int nrows, int roi, int intervis, int nsheds,
+                             const int32_t *obs, const uint64_t *sheds,
+                             uint8_t *selected, siting_stats *stats) {
+  if (!obs || !sheds || !selected || nrows <= 0 || nrows > 32767 || roi < 1 ||
+      roi > 16383 || nsheds < 1 || (intervis != 0 && intervis != 1))
+    return SITING_EINVAL;
+  const size_t n2 = (size_t)nrows * nrows;
+  for (int k = 0; k < nsheds; k++)
+    if (obs[k] < 0 || (size_t)obs[k] >= n2) return SITING_EINVAL;
+  hipStream_t s;
+  StageStream hs;
+  CHECK(stage_device(s, hs));
+  Geometry g{};
+  g.p.nrows = nrows;
+  g.p.roi = roi;
+  g.p.intervis = intervis;
+  g.d.nwanted = nsheds;
+  g.d.nr = 2 * roi + 1;
+  g.d.nwpr = (g.d.nr + 63) / 64;
+  g.d.cumnwpr = (nrows + 63) / 64;
+  g.d.shed_words = (int64_t)g.d.nr * g.d.nwpr;
+  const size_t nw = (size_t)nsheds;
+  const size_t sbytes = nw * (size_t)g.d.shed_words * 8;
+  DevBuf dobs, dsh, darea, dused, dcum, drm, dord, dst, dscr;
+  CHECK(dalloc(dobs, nw * 4));
+  CHECK(dalloc(dsh, sbytes));
+  CHECK(dalloc(darea, nw * 4));
+  CHECK(dalloc(dused, nw));
+  CHECK(dalloc(dcum, (size_t)nrows * g.d.cumnwpr * 8));
+  CHECK(dalloc(drm, 2 * (size_t)site_key_rows(nrows, roi) * 8));
+  CHECK(dalloc(dord, nw * 4));
+  CHECK(dalloc(dst, sizeof(SiteState)));
+  CHECK(dalloc(dscr, site_scratch_bytes(nrows, roi, nsheds)));
+  SITING_HIP(hipMemcpyAsync(dobs.p, obs, nw * 4, hipMemcpyHostToDevice, s));
+  SITING_HIP(hipMemcpyAsync(dsh.p, sheds, sbytes, hipMemcpyHostToDevice, s));
+  SiteBuffers b{(int32_t *)darea.p, (uint8_t *)dused.p, (uint64_t *)dcum.p,
+                (uint64_t *)drm.p, (int32_t *)dord.p, (SiteState *)dst.p,
+                (uint8_t *)dscr.p};
+  CHECK(launch_site(g, (const int32_t *)dobs.p, (const uint64_t *)dsh.p, b, 0, s));
+  SITING_HIP(hipMemcpyAsync(selected, dused.p, nw, hipMemcpyDeviceToHost, s));
+  SiteState st;
+  SITING_HIP(hipMemcpyAsync(&st, dst.p, sizeof st, hipMemcpyDeviceToHost, s));
+  SITING_HIP(hipStreamSynchronize(s));
+  if (stats) {
+    std::memset(stats, 0, sizeof *stats);
+    stats->nused = st.nused;
+    stats->niter = st.niter;
+    stats->cumarea = (int64_t)st.cumarea;
+    stats->sum_u = (int64_t)st.sum_u;
+    stats->coverage = 100.0 * (double)st.cumarea / (double)n2;
+  }
+  return SITING_OK;
+}
+
+}  // extern "C"

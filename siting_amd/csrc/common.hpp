@@ -1,0 +1,146 @@
+// common.hpp -- shared definitions of the siting HIP library (gfx950).
+//
+// Every kernel is compiled with -ffp-contract=off: the reference's
+// sequential build (g++ -O2, x86-64 baseline) never fuses a multiply and an
+// add, so no kernel may either (siting.cpp:96,106,119,129,342).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "siting.h"
+
+namespace siting {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------
+// error plumbing
+// ---------------------------------------------------------------------------
+#define SITING_HIP(call)                                   \
+  do {                                                     \
+    hipError_t e_ = (call);                                \
+    if (e_ != hipSuccess) return ::siting::hip_fail(e_, #call, __FILE__, __LINE__); \
+  } while (0)
+
+siting_status hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+// ---------------------------------------------------------------------------
+// integer helpers shared by host and device
+// ---------------------------------------------------------------------------
+
+// glibc TYPE_0 generator step on 31 bits, siting.cpp:29-33.
+__host__ __device__ inline uint32_t lcg_step(uint32_t r) {
+  return (1103515245u * (r & 0x7fffffffu) + 12345u) & 0x7fffffffu;
+}
+
+// FINDMAX tie hash h = x*(x+y)*0x41041041 with int32 wrap (siting.cpp:217).
+__host__ __device__ inline int32_t block_hash(int x, int y) {
+  return (int32_t)((uint32_t)x * (uint32_t)(x + y) * 0x41041041u);
+}
+
+// Floor split of a (possibly negative) column into a 64-bit word index and a
+// bit shift (siting.cpp:400-405).
+__host__ __device__ inline void col_split(int c, int &word, int &bit) {
+  int w = c / 64, b = c % 64;
+  if (b < 0) {
+    w -= 1;
+    b += 64;
+  }
+  word = w;
+  bit = b;
+}
+
+// Round half away from zero, as libm round() (siting.cpp:95,330).  x - t is
+// exact for |x| < 2^23, so the comparison against 0.5 is exact.
+__device__ __forceinline__ int round_half_away(float x) {
+  float t = __builtin_truncf(x);
+  float f = x - t;
+  int r = (int)t;
+  r += (f >= 0.5f) ? 1 : 0;
+  r -= (f <= -0.5f) ? 1 : 0;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// wave / block reductions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    uint64_t o = __shfl_xor(v, off, kWave);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ int wave_sum_i32(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// device-side SITE state (one per job)
+// ---------------------------------------------------------------------------
+struct SiteState {
+  unsigned long long cumarea;
+  unsigned long long sum_u;
+  int nused;
+  int niter;
+  unsigned int barrier;  // grid barrier arrival counter
+  int pad;
+};
+
+// ---------------------------------------------------------------------------
+// job geometry (host side), derived once per job
+// ---------------------------------------------------------------------------
+struct Geometry {
+  siting_params p;
+  siting_derived d;
+};
+
+siting_status derive(const siting_params *p, siting_derived *d);
+
+// ---------------------------------------------------------------------------
+// kernel launchers (one per stage), all asynchronous on `stream`
+// ---------------------------------------------------------------------------
+struct VixTables {
+  const float *recip;      // (float)(1.0/d) for d = 0..roi (entry 0 unused)
+  const uint32_t *jump;    // LCG jump-ahead: (A_m, C_m) for m = 1..2*ntests
+};
+
+siting_status launch_vix(const Geometry &g, const uint16_t *elevs, uint8_t *vix,
+                         const VixTables &t, hipStream_t s);
+siting_status launch_findmax(const Geometry &g, uint8_t *vix, int32_t *obs,
+                             hipStream_t s);
+siting_status launch_viewshed(const Geometry &g, const uint16_t *elevs,
+                              const int32_t *obs, uint64_t *sheds,
+                              hipStream_t s);
+
+struct SiteBuffers {
+  int32_t *area;      // [nwanted] testshedarea
+  uint8_t *used;      // [nwanted] usedq == selected
+  uint64_t *cum;      // [nrows * cumnwpr] cumulative bitset
+  uint64_t *rowmax;   // [2 * nblocks] double-buffered per-block-row argmax keys
+  int32_t *order;     // [nwanted] selection order
+  SiteState *state;
+  uint8_t *scratch;   // site_scratch_bytes(): candidate buckets
+};
+
+// Device scratch SITE needs beyond SiteBuffers' arrays, and the size of the
+// double-buffered per-row key array (entries).
+size_t site_scratch_bytes(int nrows, int roi, int nsheds);
+int site_key_rows(int nrows, int roi);
+
+siting_status launch_site(const Geometry &g, const int32_t *obs,
+                          const uint64_t *sheds, const SiteBuffers &b,
+                          int device, hipStream_t s);
+
+// host tables for VIX
+void fill_recip_table(int roi, float *out);               // roi+1 entries
+void fill_jump_table(int ntests, uint32_t *out);          // 4*ntests entries
+
+}  // namespace siting

@@ -1,0 +1,139 @@
+// viewshed.hip -- VIEWSHED stage: one exact radial-sweep bitset per
+// tentative observer (siting.cpp:243-360; siting.cu:250-370 has the same
+// arithmetic).
+//
+// A workgroup owns one observer.  Its lanes take the 8*roi rays to the cells
+// of the box perimeter (each corner once, siting.cpp:292-304); adjacent lanes
+// take adjacent perimeter cells, so at every step the wavefront reads a
+// short arc of neighbouring DEM cells.  Along a ray the running horizon
+// slope is kept in float exactly as the reference computes it: IEEE
+// divisions for the ray slope and the point slope, round-half-away for the
+// minor coordinate, no fused multiply-add (siting.cpp:323-344).  Visible
+// cells are OR'ed into the shed bitset held in LDS (MSB-first 64-bit words,
+// siting.cpp:247-250); the finished bitset goes to HBM with coalesced
+// 64-bit stores, zero words included, so no separate clear is needed.  When
+// the bitset does not fit the LDS budget (roi above ~350) the rays OR
+// straight into HBM with device atomics after the workgroup clears it.
+//
+// Observers are assigned XCD-major (workgroup b runs on XCD b%8, each XCD
+// takes a contiguous run of observers, which FINDMAX emitted block by block,
+// i.e. spatially adjacent), so each XCD's L2 serves one band of the DEM.
+#include "common.hpp"
+
+namespace siting {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kLdsWords = 8192;  // 64 KiB bitset budget
+
+template <bool kInLds>
+__global__ __launch_bounds__(kThreads) void viewshed_kernel(
+    const uint16_t *__restrict__ elevs, const int32_t *__restrict__ obs,
+    uint64_t *__restrict__ sheds, int n, int roi, int oht, int tht, int nwpr,
+    int nsheds, int per_xcd) {
+  extern __shared__ uint64_t lds_bits[];
+  const int b = blockIdx.x;
+  const int k = (b & 7) * per_xcd + (b >> 3);
+  if (k >= nsheds) return;
+  const int nr = 2 * roi + 1;
+  const int words = nr * nwpr;
+  uint64_t *out = sheds + (size_t)k * words;
+  uint64_t *bits = kInLds ? lds_bits : out;
+
+  for (int i = threadIdx.x; i < words; i += kThreads) bits[i] = 0;
+  if (!kInLds) __threadfence();
+  __syncthreads();
+
+  const int o = obs[k];
+  const int ox = o / n, oy = o - (o / n) * n;
+  const int oalt = (int)elevs[o] + oht;
+  const float foalt = (float)oalt;
+  const int rr = roi * roi;
+  const int twor = 2 * roi;
+
+  if (threadIdx.x == 0) {  // observer sees itself, siting.cpp:279
+    const uint64_t m = 1ull << (63 - (roi & 63));
+    if (kInLds)
+      atomicOr((unsigned long long *)&bits[roi * nwpr + (roi >> 6)], m);
+    else
+      atomicOr((unsigned long long *)&bits[roi * nwpr + (roi >> 6)], m);
+  }
+
+  for (int ray = threadIdx.x; ray < 8 * roi; ray += kThreads) {
+    const int side = ray / twor;
+    const int j = ray - side * twor;
+    int dx, dy;
+    if (side == 0) {
+      dx = -roi + j;
+      dy = -roi;
+    } else if (side == 1) {
+      dx = -roi + 1 + j;
+      dy = roi;
+    } else if (side == 2) {
+      dx = -roi;
+      dy = -roi + 1 + j;
+    } else {
+      dx = roi;
+      dy = -roi + j;
+    }
+    const int adx = dx < 0 ? -dx : dx, ady = dy < 0 ? -dy : dy;
+    const bool along_y = adx < ady;
+    const int major = along_y ? dy : dx;
+    const int minor = along_y ? dx : dy;
+    const int len = along_y ? ady : adx;
+    const int dir = major > 0 ? 1 : -1;
+    const float slope = (float)minor / (float)major;  // siting.cpp:323
+    float horizon = -99999.0f;                          // siting.cpp:325
+    for (int a = 1; a <= len; a++) {
+      const int i = dir * a;
+      const int off = round_half_away((float)i * slope);  // siting.cpp:330
+      const int qx = along_y ? off : i;
+      const int qy = along_y ? i : off;
+      const int px = ox + qx, py = oy + qy;
+      if (px < 0 || px >= n || py < 0 || py >= n) break;  // siting.cpp:333
+      if (qx * qx + qy * qy > rr) break;                  // siting.cpp:334
+      const int pel = (int)elevs[px * n + py];
+      const float fa = (float)a;
+      const float sl = (float)(pel - oalt) / fa;  // siting.cpp:340
+      if (horizon < sl) horizon = sl;
+      const float rise = horizon * fa;
+      const float halt = foalt + rise;  // siting.cpp:342
+      if ((float)(pel + tht) >= halt) {
+        const int row = qx + roi, col = qy + roi;
+        atomicOr((unsigned long long *)&bits[row * nwpr + (col >> 6)],
+                 1ull << (63 - (col & 63)));
+      }
+    }
+  }
+
+  if (kInLds) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < words; i += kThreads) out[i] = bits[i];
+  }
+}
+
+}  // namespace
+
+siting_status launch_viewshed(const Geometry &g, const uint16_t *elevs,
+                              const int32_t *obs, uint64_t *sheds,
+                              hipStream_t s) {
+  const int nsheds = g.d.nwanted;
+  if (nsheds <= 0) return SITING_OK;
+  const int per_xcd = (nsheds + 7) / 8;
+  const int words = g.d.nr * g.d.nwpr;
+  if (words <= kLdsWords) {
+    hipLaunchKernelGGL(viewshed_kernel<true>, dim3(8 * per_xcd), dim3(kThreads),
+                       sizeof(uint64_t) * words, s, elevs, obs, sheds,
+                       g.p.nrows, g.p.roi, g.p.oht, g.p.tht, g.d.nwpr, nsheds,
+                       per_xcd);
+  } else {
+    hipLaunchKernelGGL(viewshed_kernel<false>, dim3(8 * per_xcd), dim3(kThreads),
+                       0, s, elevs, obs, sheds, g.p.nrows, g.p.roi, g.p.oht,
+                       g.p.tht, g.d.nwpr, nsheds, per_xcd);
+  }
+  SITING_HIP(hipGetLastError());
+  return SITING_OK;
+}
+
+}  // namespace siting
