@@ -1,0 +1,342 @@
+"""MI355X-native multiple-observer siting (VIX -> FINDMAX -> VIEWSHED -> SITE).
+
+Python view of the C ABI in ``include/siting.h`` (``lib/libsiting_amd.so``).
+The four stage functions keep the reference's names, argument order and
+meaning (siting.cpp:192, 235, 349, 501); they take and return numpy arrays
+and raise :class:`SitingError` where the reference would ``die()``.
+
+There is no CPU fallback: importing works without a GPU, but every compute
+call needs the HIP library and a device, and fails loudly otherwise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libsiting_amd.so")
+CLI_PATH = os.path.join(_HERE, "bin", "hipsite")
+
+OK, EINVAL, ERANGE, ESMALLBLOCK, ENOMEM, EDEVICE, EIO, EBUSY, ENOTFOUND = range(9)
+
+BUF_ELEVS, BUF_VIX, BUF_OBS, BUF_SHEDS, BUF_SELECTED, BUF_ORDER = range(6)
+
+
+class SitingError(RuntimeError):
+    """A non-OK siting_status from the library."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"siting status {status}: {message}")
+        self.status = status
+        self.message = message
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("nrows", ctypes.c_int),
+        ("roi", ctypes.c_int),
+        ("oht", ctypes.c_int),
+        ("tht", ctypes.c_int),
+        ("ntests", ctypes.c_int),
+        ("bw", ctypes.c_int),
+        ("nwanted0", ctypes.c_int),
+        ("intervis", ctypes.c_int),
+    ]
+
+
+class Derived(ctypes.Structure):
+    _fields_ = [
+        ("blocksize", ctypes.c_float),
+        ("nblocks", ctypes.c_int),
+        ("nwpb", ctypes.c_int),
+        ("nwanted", ctypes.c_int),
+        ("lastsize", ctypes.c_int),
+        ("nr", ctypes.c_int),
+        ("nwpr", ctypes.c_int),
+        ("cumnwpr", ctypes.c_int),
+        ("shed_words", ctypes.c_int64),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("nused", ctypes.c_int),
+        ("niter", ctypes.c_int),
+        ("cumarea", ctypes.c_int64),
+        ("sum_u", ctypes.c_int64),
+        ("coverage", ctypes.c_double),
+        ("ms_vix", ctypes.c_float),
+        ("ms_findmax", ctypes.c_float),
+        ("ms_viewshed", ctypes.c_float),
+        ("ms_site", ctypes.c_float),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_lib = None
+
+_VP = ctypes.c_void_p
+_I = ctypes.c_int
+_SIGS = {
+    "siting_status_str": (ctypes.c_char_p, [_I]),
+    "siting_derive": (_I, [ctypes.POINTER(Params), ctypes.POINTER(Derived)]),
+    "siting_check_cli": (_I, [ctypes.POINTER(Params), ctypes.POINTER(_I),
+                             ctypes.POINTER(ctypes.c_char_p)]),
+    "siting_calc_vix": (_I, [_I, _VP, _I, _I, _I, _I, _VP]),
+    "siting_find_max": (_I, [_I, _VP, ctypes.c_float, _I, _I, _I, _VP]),
+    "siting_calc_sheds": (_I, [_I, _VP, _I, _I, _I, _I, _VP, _VP]),
+    "siting_site_it": (_I, [_I, _I, _I, _I, _VP, _VP, _VP, ctypes.POINTER(Stats)]),
+    "siting_job_bytes": (_I, [ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_size_t)]),
+    "siting_job_create": (_I, [_I, ctypes.POINTER(Params), ctypes.POINTER(_VP)]),
+    "siting_job_destroy": (None, [_VP]),
+    "siting_job_upload_dem": (_I, [_VP, _VP]),
+    "siting_job_vix": (_I, [_VP]),
+    "siting_job_findmax": (_I, [_VP]),
+    "siting_job_viewshed": (_I, [_VP]),
+    "siting_job_site": (_I, [_VP]),
+    "siting_job_run": (_I, [_VP]),
+    "siting_job_sync": (_I, [_VP]),
+    "siting_job_download": (_I, [_VP, _I, _VP, ctypes.c_size_t]),
+    "siting_job_device_ptr": (_I, [_VP, _I, ctypes.POINTER(_VP),
+                                   ctypes.POINTER(ctypes.c_size_t)]),
+    "siting_job_stats": (_I, [_VP, ctypes.POINTER(Stats)]),
+    "siting_job_derived": (_I, [_VP, ctypes.POINTER(Derived)]),
+    "siting_dem_read": (_I, [ctypes.c_char_p, _I, _VP]),
+    "siting_dem_write": (_I, [ctypes.c_char_p, _I, _VP]),
+    "siting_dem_generate": (_I, [_I, ctypes.c_double, _I, _I, _I, ctypes.c_double,
+                                 ctypes.c_double, ctypes.c_uint64, _VP]),
+    "siting_device_count": (_I, []),
+}
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SitingError(EDEVICE, f"{LIB_PATH} is not built "
+                              "(run __graft_entry__.build())")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def _check(status: int) -> None:
+    if status != OK:
+        msg = lib().siting_status_str(status).decode()
+        raise SitingError(status, msg)
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def device_count() -> int:
+    return lib().siting_device_count()
+
+
+def make_params(nrows, roi, height=None, ntests=None, bw=None, nwanted0=None,
+                intervis=0, oht=None, tht=None) -> Params:
+    oht = height if oht is None else oht
+    tht = height if tht is None else tht
+    return Params(nrows, roi, oht, tht, ntests, bw, nwanted0, intervis)
+
+
+def derive(p: Params) -> Derived:
+    """siting.cpp:689-702 (no CLI range checks)."""
+    d = Derived()
+    _check(lib().siting_derive(ctypes.byref(p), ctypes.byref(d)))
+    return d
+
+
+def check_cli(p: Params) -> tuple[int, str]:
+    """siting.cpp:680-686: (0, "") or (index, die message)."""
+    which = ctypes.c_int(0)
+    msg = ctypes.c_char_p()
+    lib().siting_check_cli(ctypes.byref(p), ctypes.byref(which), ctypes.byref(msg))
+    return which.value, (msg.value or b"").decode()
+
+
+# ---- stage functions (the reference's signatures) -------------------------
+
+def calc_vix(nrows, elevs, roi, oht, tht, ntests) -> np.ndarray:
+    """siting.cpp:192 calc_vix: uint8[nrows*nrows] visibility index."""
+    e = np.ascontiguousarray(elevs, dtype=np.uint16).reshape(-1)
+    if e.size != nrows * nrows:
+        raise SitingError(EINVAL, "elevs must hold nrows*nrows values")
+    vix = np.empty(nrows * nrows, np.uint8)
+    _check(lib().siting_calc_vix(nrows, _ptr(e), roi, oht, tht, ntests, _ptr(vix)))
+    return vix
+
+
+def find_max(nrows, vix, blocksize, nwanted, nblocks, nwpb) -> np.ndarray:
+    """siting.cpp:235 find_max: obs[nblocks^2 * nwpb]; vix is mutated in place
+    (picked cells set to 0) as in the sequential reference."""
+    if not (isinstance(vix, np.ndarray) and vix.dtype == np.uint8
+            and vix.flags.c_contiguous and vix.size == nrows * nrows):
+        raise SitingError(EINVAL, "vix must be a contiguous uint8 array of nrows^2")
+    obs = np.empty(nblocks * nblocks * nwpb, np.int32)
+    _check(lib().siting_find_max(nrows, _ptr(vix), blocksize, nwanted, nblocks,
+                                 nwpb, _ptr(obs)))
+    return obs
+
+
+def calc_sheds(nrows, elevs, roi, oht, tht, nsheds, obs) -> np.ndarray:
+    """siting.cpp:349 calc_sheds: uint64[nsheds * nr * nwpr] MSB-first."""
+    e = np.ascontiguousarray(elevs, dtype=np.uint16).reshape(-1)
+    o = np.ascontiguousarray(obs, dtype=np.int32)
+    nr = 2 * roi + 1
+    nwpr = (nr + 63) // 64
+    if e.size != nrows * nrows or o.size < nsheds:
+        raise SitingError(EINVAL, "bad elevs/obs size")
+    sheds = np.empty(nsheds * nr * nwpr, np.uint64)
+    _check(lib().siting_calc_sheds(nrows, _ptr(e), roi, oht, tht, nsheds, _ptr(o),
+                                   _ptr(sheds)))
+    return sheds
+
+
+def site_it(nrows, roi, intervis, nsheds, obs, sheds):
+    """siting.cpp:501 site_it: (selected uint8[nsheds], Stats)."""
+    o = np.ascontiguousarray(obs, dtype=np.int32)
+    s = np.ascontiguousarray(sheds, dtype=np.uint64)
+    nr = 2 * roi + 1
+    nwpr = (nr + 63) // 64
+    if o.size < nsheds or s.size < nsheds * nr * nwpr:
+        raise SitingError(EINVAL, "bad obs/sheds size")
+    sel = np.empty(nsheds, np.uint8)
+    st = Stats()
+    _check(lib().siting_site_it(nrows, roi, intervis, nsheds, _ptr(o), _ptr(s),
+                                _ptr(sel), ctypes.byref(st)))
+    return sel, st
+
+
+# ---- device-resident job ---------------------------------------------------
+
+class Job:
+    """One siting job whose arrays stay in HBM across the four stages."""
+
+    def __init__(self, params: Params, device: int = 0):
+        self.params = params
+        h = ctypes.c_void_p()
+        _check(lib().siting_job_create(device, ctypes.byref(params), ctypes.byref(h)))
+        self._h = h
+        d = Derived()
+        _check(lib().siting_job_derived(self._h, ctypes.byref(d)))
+        self.derived = d
+
+    @staticmethod
+    def bytes_needed(params: Params) -> int:
+        b = ctypes.c_size_t()
+        _check(lib().siting_job_bytes(ctypes.byref(params), ctypes.byref(b)))
+        return b.value
+
+    def close(self):
+        if self._h:
+            lib().siting_job_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload_dem(self, elevs):
+        e = np.ascontiguousarray(elevs, dtype=np.uint16).reshape(-1)
+        if e.size != self.params.nrows ** 2:
+            raise SitingError(EINVAL, "DEM size mismatch")
+        _check(lib().siting_job_upload_dem(self._h, _ptr(e)))
+
+    def vix(self):
+        _check(lib().siting_job_vix(self._h))
+
+    def findmax(self):
+        _check(lib().siting_job_findmax(self._h))
+
+    def viewshed(self):
+        _check(lib().siting_job_viewshed(self._h))
+
+    def site(self):
+        _check(lib().siting_job_site(self._h))
+
+    def run(self):
+        _check(lib().siting_job_run(self._h))
+
+    def sync(self):
+        _check(lib().siting_job_sync(self._h))
+
+    def stats(self) -> Stats:
+        st = Stats()
+        _check(lib().siting_job_stats(self._h, ctypes.byref(st)))
+        return st
+
+    def _download(self, which, dtype, count):
+        out = np.empty(count, dtype)
+        _check(lib().siting_job_download(self._h, which, _ptr(out), out.nbytes))
+        return out
+
+    def download_vix(self):
+        return self._download(BUF_VIX, np.uint8, self.params.nrows ** 2)
+
+    def download_obs(self):
+        return self._download(BUF_OBS, np.int32, self.derived.nwanted)
+
+    def download_sheds(self):
+        return self._download(BUF_SHEDS, np.uint64,
+                              self.derived.nwanted * self.derived.shed_words)
+
+    def download_selected(self):
+        return self._download(BUF_SELECTED, np.uint8, self.derived.nwanted)
+
+    def download_order(self, nused: int):
+        return self._download(BUF_ORDER, np.int32, nused)
+
+
+# ---- terrain ---------------------------------------------------------------
+
+@dataclass
+class TerrainSpec:
+    """Seeded synthetic DEM: diamond-square roughness H, linear rescale to
+    [hmin, hmax], plus Gaussian hills."""
+    nrows: int
+    rough_h: float = 0.8
+    hmin: int = 0
+    hmax: int = 20000
+    nhills: int = 8
+    hill_peak: float = 5000.0
+    hill_sigma: float = 80.0
+    seed: int = 1
+
+
+def dem_generate(spec: TerrainSpec) -> np.ndarray:
+    out = np.empty(spec.nrows * spec.nrows, np.uint16)
+    _check(lib().siting_dem_generate(spec.nrows, spec.rough_h, spec.hmin, spec.hmax,
+                                     spec.nhills, spec.hill_peak, spec.hill_sigma,
+                                     spec.seed & (2 ** 64 - 1), _ptr(out)))
+    return out
+
+
+def dem_read(path: str, nrows: int) -> np.ndarray:
+    out = np.empty(nrows * nrows, np.uint16)
+    _check(lib().siting_dem_read(path.encode(), nrows, _ptr(out)))
+    return out
+
+
+def dem_write(path: str, nrows: int, dem: np.ndarray) -> None:
+    e = np.ascontiguousarray(dem, dtype=np.uint16).reshape(-1)
+    _check(lib().siting_dem_write(path.encode(), nrows, _ptr(e)))
