@@ -17,7 +17,7 @@
 //   site_first_pass area = popcount(shed) for every candidate (cum is empty,
 //                   no mask), streaming the whole shed array once
 //   site_rowmax_init  per cell row, the best (area, -index) key
-//   site_loop       persistent cooperative kernel, one grid barrier per
+//   site_loop       persistent kernel, one grid barrier per
 //                   iteration.  Each workgroup owns the cell rows r with
 //                   r % G == g.  Iteration: every workgroup reduces the
 //                   per-row keys to the winner w (identically), stages w's
@@ -456,27 +456,29 @@ siting_status launch_site(const Geometry &g, const int32_t *obs,
   hipLaunchKernelGGL(site_rowmax_init, dim3(a.ncr), dim3(256), 0, s, a);
   SITING_HIP(hipGetLastError());
 
-  // persistent loop: as many workgroups as cell rows an update window can
-  // touch, capped by what is co-resident (one 1024-thread group per CU)
+  // Persistent loop: as many workgroups as cell rows an update window can
+  // touch, and at most half the CUs.  The grid barrier needs every
+  // workgroup resident at once; a grid this far below the device's capacity
+  // (one 1024-thread group per CU) is, since any other kernel that holds CUs
+  // when it starts does not wait on it and retires.  A plain launch is used:
+  // a cooperative launch reserves a device-wide resource that a second
+  // process on the same GPU cannot get.
   const int touched = (4 * roi) / a.cs + 2;
   int cus = 0;
   SITING_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
                                    device));
   const bool lds_delta = (size_t)g.d.nr * (g.d.nwpr + 1) <= (size_t)kDeltaLdsWords;
   const size_t shmem = lds_delta ? sizeof(uint64_t) * g.d.nr * (g.d.nwpr + 1) : 0;
-  const void *fn = lds_delta ? (const void *)site_loop<true>
-                             : (const void *)site_loop<false>;
-  int per_cu = 0;
-  SITING_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn,
-                                                          kLoopThreads, shmem));
   int grid = touched;
   grid = grid < a.ncr ? grid : a.ncr;
-  const int cap = cus * (per_cu > 0 ? per_cu : 1);
+  const int cap = cus / 2 > 0 ? cus / 2 : 1;
   grid = grid < cap ? grid : cap;
   grid = grid < 1 ? 1 : grid;
-  void *args[] = {&a};
-  SITING_HIP(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kLoopThreads), args,
-                                        shmem, s));
+  if (lds_delta)
+    hipLaunchKernelGGL(site_loop<true>, dim3(grid), dim3(kLoopThreads), shmem, s, a);
+  else
+    hipLaunchKernelGGL(site_loop<false>, dim3(grid), dim3(kLoopThreads), 0, s, a);
+  SITING_HIP(hipGetLastError());
   return SITING_OK;
 }
 
