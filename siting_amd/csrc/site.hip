@@ -49,6 +49,7 @@ constexpr int kLoopWaves = kLoopThreads / kWave;
 constexpr int kMaxList = 1024;
 constexpr size_t kLdsBudget = 140 * 1024;  // dynamic LDS of the loop kernel
 constexpr int kMaxGroups = 128;
+constexpr int kMaxBatch = 16;  // picks per round (intervis 0)
 
 struct SiteArgs {
   const int32_t *obs;
@@ -56,8 +57,8 @@ struct SiteArgs {
   int32_t *area;     // [nsheds] first-pass area by candidate id
   uint8_t *used;     // [nsheds] selected flags (the output mask)
   uint64_t *cum;     // [n * cw]
-  uint64_t *keys;    // [2][nunits]
-  uint32_t *kpos;    // [2][nunits]
+  uint64_t *keys;    // [2][nunits][2] best two keys per unit
+  uint32_t *kpos;    // [2][nunits][2] their positions
   int32_t *order;    // [nsheds]
   SiteState *st;
   int32_t *bcount;   // [nbuckets] counts, then scatter cursors
@@ -70,7 +71,8 @@ struct SiteArgs {
   int n, roi, nr, nwpr, cw, nsheds, intervis;
   int cs, ncr, Q, nunits, G, upw, kpu, nbuckets, cap;
   // dynamic LDS layout (byte offsets)
-  int o_delta, o_ukey, o_upos, o_dirty, o_off, o_c, o_pos, o_t, o_used;
+  int o_delta, o_ukey, o_upos, o_dirty, o_off, o_c, o_pos, o_t, o_used, o_sk, o_sp,
+      o_stk;
   int off_lds;  // unit cell offsets kept in LDS (else read from bstart)
   double npts;
 };
@@ -246,11 +248,11 @@ struct Winner {
 // word) pairs inside the terrain; lane l takes items l, l+64, ..., so a load
 // instruction reads consecutive words of the shed (rows are contiguous).
 // kBatch items per lane are loaded before any is used.
-constexpr int kBatch = 8;
+constexpr int kBatch = 4;
 
 __device__ __forceinline__ int wave_new_bits(const uint64_t *sh, int cx, int cy,
                                              const SiteArgs &a, const Winner &W,
-                                             int lane) {
+                                             int lane, bool use_w) {
   const int roi = a.roi, nwpr = a.nwpr, cw = a.cw, n = a.n;
   int w0c, fbc;
   col_split(cy - roi, w0c, fbc);
@@ -278,7 +280,8 @@ __device__ __forceinline__ int wave_new_bits(const uint64_t *sh, int cx, int cy,
       prv[u] = (ok && k > 0) ? srow[k - 1] : 0;
       uint64_t c = ok ? a.cum[(size_t)(xo + row) * cw + (w0c + k)] : ~0ull;
       const int dr = dro + row, dk = w0c + k - W.w0w;
-      if (ok && dr >= 0 && dr < W.nr && dk >= 0 && dk <= W.nwpr) c |= W.word(dr, dk);
+      if (use_w && ok && dr >= 0 && dr < W.nr && dk >= 0 && dk <= W.nwpr)
+        c |= W.word(dr, dk);
       cv[u] = c;
     }
 #pragma unroll
@@ -291,28 +294,66 @@ __device__ __forceinline__ int wave_new_bits(const uint64_t *sh, int cx, int cy,
   return wave_sum_i32(s);
 }
 
-template <int MW>
+// Best and second best (key, position) over the unused candidates of owned
+// unit ul; one wavefront.
+struct Top2 {
+  uint64_t k1, k2;
+  uint32_t p1, p2;
+};
+
+__device__ __forceinline__ void top2_push(Top2 &t, uint64_t k, uint32_t p) {
+  if (k > t.k1) {
+    t.k2 = t.k1;
+    t.p2 = t.p1;
+    t.k1 = k;
+    t.p1 = p;
+  } else if (k > t.k2) {
+    t.k2 = k;
+    t.p2 = p;
+  }
+}
+
+__device__ __forceinline__ Top2 wave_top2(Top2 t) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    Top2 u;
+    u.k1 = __shfl_xor(t.k1, o, kWave);
+    u.k2 = __shfl_xor(t.k2, o, kWave);
+    u.p1 = __shfl_xor(t.p1, o, kWave);
+    u.p2 = __shfl_xor(t.p2, o, kWave);
+    top2_push(t, u.k1, u.p1);
+    top2_push(t, u.k2, u.p2);
+  }
+  return t;
+}
+
+template <bool kDeltaLds>
 __global__ __launch_bounds__(kLoopThreads) void site_loop(SiteArgs a) {
   extern __shared__ __align__(16) uint8_t smem[];
-  __shared__ uint64_t redk[kLoopWaves];
-  __shared__ uint32_t redp[kLoopWaves];
   __shared__ int list_i[kMaxList];
   __shared__ int list_acc[kMaxList];
   __shared__ int list_n;
+  __shared__ uint64_t bw_key[kMaxBatch];
+  __shared__ uint32_t bw_pos[kMaxBatch];
+  __shared__ int bw_n;
 
-  constexpr bool kDeltaLds = MW > 0;
   const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
   const int lane = tid & 63, wid = tid >> 6;
   const int n = a.n, roi = a.roi, nr = a.nr, nwpr = a.nwpr, cw = a.cw;
-  const int Q = a.Q, upw = a.upw, kpu = a.kpu;
+  const int Q = a.Q, upw = a.upw, kpu = a.kpu, nunits = a.nunits;
   const int r2 = (2 * roi) * (2 * roi);
   const size_t S = (size_t)nr * nwpr;
+  const int dw = nwpr + 1;
+  const int kmax = a.intervis ? 1 : kMaxBatch;
 
   uint64_t *delta = (uint64_t *)(smem + a.o_delta);
-  uint64_t *ukey = (uint64_t *)(smem + a.o_ukey);
-  uint32_t *upos = (uint32_t *)(smem + a.o_upos);
-  uint8_t *dirty = smem + a.o_dirty;
+  uint64_t *ukey = (uint64_t *)(smem + a.o_ukey);   // [upw][2]
+  uint32_t *upos = (uint32_t *)(smem + a.o_upos);   // [upw][2]
+  uint8_t *dirty = smem + a.o_dirty;                 // [upw]
   int32_t *off_l = (int32_t *)(smem + a.o_off);
+  uint64_t *sk = (uint64_t *)(smem + a.o_sk);       // [nunits][2] staged keys
+  uint32_t *sp = (uint32_t *)(smem + a.o_sp);       // [nunits][2] staged positions
+  uint8_t *stk = smem + a.o_stk;                     // [nunits] picks taken
 
   // ---- this workgroup's candidates
   const int B0 = a.bstart[(size_t)g * upw * kpu];
@@ -343,219 +384,285 @@ __global__ __launch_bounds__(kLoopThreads) void site_loop(SiteArgs a) {
       off_l[e] = bst[(size_t)ul * kpu + k] - B0;
     }
   if (tid == 0) list_n = 0;
+  for (int ul = tid; ul < upw; ul += kLoopThreads) dirty[ul] = 0;
   __syncthreads();
 
-  // best key of owned unit ul -> ukey/upos (one wavefront per unit)
-  auto unit_key = [&](int ul, int w) {
+  auto unit_top2 = [&](int ul) {
     const int lo = OFF(ul, 0), hi = OFF(ul, kpu);
-    uint64_t best = 0;
-    uint32_t bpos = 0;
-    for (int i = lo + lane; i < hi; i += kWave) {
-      const int c = C[i];
-      if (U[i] || c == w) continue;
-      const uint64_t k = cand_key(T[i], c);
-      if (k > best) {
-        best = k;
-        bpos = P[i];
-      }
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const uint64_t ok = __shfl_xor(best, o, kWave);
-      const uint32_t op = __shfl_xor(bpos, o, kWave);
-      if (ok > best) {
-        best = ok;
-        bpos = op;
-      }
-    }
+    Top2 t{0, 0, 0, 0};
+    for (int i = lo + lane; i < hi; i += kWave)
+      if (!U[i]) top2_push(t, cand_key(T[i], C[i]), P[i]);
+    t = wave_top2(t);
     if (lane == 0) {
-      ukey[ul] = best;
-      upos[ul] = bpos;
+      ukey[2 * ul] = t.k1;
+      ukey[2 * ul + 1] = t.k2;
+      upos[2 * ul] = t.p1;
+      upos[2 * ul + 1] = t.p2;
+    }
+  };
+  auto publish = [&](int parity) {
+    uint64_t *kn = a.keys + (size_t)parity * nunits * 2;
+    uint32_t *pn = a.kpos + (size_t)parity * nunits * 2;
+    for (int e = tid; e < 2 * upw; e += kLoopThreads) {
+      const int u = (e >> 1) * G + g;
+      if (u < nunits) {
+        kn[2 * u + (e & 1)] = ukey[e];
+        pn[2 * u + (e & 1)] = upos[e];
+      }
     }
   };
 
-  for (int ul = wid; ul < upw; ul += kLoopWaves) unit_key(ul, -1);
+  for (int ul = wid; ul < upw; ul += kLoopWaves) unit_top2(ul);
   __syncthreads();
-  for (int ul = tid; ul < upw; ul += kLoopThreads) {
-    const int u = ul * G + g;
-    if (u < a.nunits) {
-      a.keys[u] = ukey[ul];
-      a.kpos[u] = upos[ul];
-    }
-  }
+  publish(0);
   unsigned int bar_target = 0;
   grid_sync(&a.st->barrier, bar_target, (unsigned)G);
 
   int nused = 0, niter = 1, parity = 0;
   long long cumarea = 0, my_u = 0;
-  const int dw = nwpr + 1;
 
   for (;;) {
-    // ---- argmax over the unit keys (identical in every workgroup)
-    const uint64_t *kk = a.keys + (size_t)parity * a.nunits;
-    const uint32_t *kp = a.kpos + (size_t)parity * a.nunits;
-    uint64_t m = 0;
-    uint32_t mp = 0;
-    for (int i = tid; i < a.nunits; i += kLoopThreads) {
-      const uint64_t v = kk[i];
-      if (v > m) {
-        m = v;
-        mp = kp[i];
+    // ---- stage the unit tops and pick the round's winners (identical in
+    // every workgroup).  Winners are a prefix of the global key order in
+    // which no two are within 2*roi of each other: none of them is in
+    // another's update list, so the sequential loop would pick them in
+    // this order (areas only shrink when intervis is 0).  A unit whose two
+    // published candidates are both taken ends the prefix (its third is
+    // unknown).
+    {
+      const uint64_t *kk = a.keys + (size_t)parity * nunits * 2;
+      const uint32_t *kp = a.kpos + (size_t)parity * nunits * 2;
+      for (int e = tid; e < 2 * nunits; e += kLoopThreads) {
+        sk[e] = kk[e];
+        sp[e] = kp[e];
       }
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const uint64_t ok = __shfl_xor(m, o, kWave);
-      const uint32_t op = __shfl_xor(mp, o, kWave);
-      if (ok > m) {
-        m = ok;
-        mp = op;
-      }
-    }
-    if (lane == 0) {
-      redk[wid] = m;
-      redp[wid] = mp;
+      for (int u = tid; u < nunits; u += kLoopThreads) stk[u] = 0;
     }
     __syncthreads();
-    m = 0;
-    mp = 0;
-    for (int i = 0; i < kLoopWaves; i++)
-      if (redk[i] > m) {
-        m = redk[i];
-        mp = redp[i];
+    if (wid == 0) {
+      int k = 0;
+      while (k < kmax) {
+        uint64_t bv = 0;
+        int bu = -1;
+        for (int u = lane; u < nunits; u += kWave) {
+          const int t = stk[u];
+          const uint64_t v = sk[2 * u + (t == 0 ? 0 : 1)];
+          if (v > bv) {
+            bv = v;
+            bu = u;
+          }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+          const uint64_t ov = __shfl_xor(bv, o, kWave);
+          const int ou = __shfl_xor(bu, o, kWave);
+          if (ov > bv) {
+            bv = ov;
+            bu = ou;
+          }
+        }
+        if ((bv >> 32) == 0) break;       // no candidate adds area
+        const int t = stk[bu];
+        if (t >= 2) break;                // the unit's next is unknown
+        const uint32_t pos = sp[2 * bu + t];
+        const int px = (int)(pos >> 16), py = (int)(pos & 0xffffu);
+        bool near = false;
+        if (lane < k) {
+          const int ex = px - (int)(bw_pos[lane] >> 16);
+          const int ey = py - (int)(bw_pos[lane] & 0xffffu);
+          near = ex * ex + ey * ey <= r2;
+        }
+        if (__any(near)) break;
+        if (lane == 0) {
+          bw_key[k] = bv;
+          bw_pos[k] = pos;
+          stk[bu] = (uint8_t)(t + 1);
+        }
+        k++;
+        __builtin_amdgcn_wave_barrier();
       }
-    const int tw = (int)(m >> 32);
-    if (tw == 0) break;  // cpp:589-592
-    if (nused >= a.nsheds) {  // cannot happen: every pick retires a candidate
+      if (lane == 0) bw_n = k;
+    }
+    __syncthreads();
+    int k = bw_n;
+    if (k == 0) break;  // cpp:589-592
+    if (nused + k > a.nsheds) {  // cannot happen: every pick retires a candidate
       niter = -1;
       break;
     }
-    const int w = (int)(~(uint32_t)m);
-    nused++;
-    cumarea += tw;
-    if (g == 0 && tid == 0) {
-      a.used[w] = 1;
-      a.order[nused - 1] = w;
-    }
-    Winner W;
-    W.w = w;
-    W.wx = (int)(mp >> 16);
-    W.wy = (int)(mp & 0xffffu);
-    col_split(W.wy - roi, W.w0w, W.fbw);
-    W.nr = nr;
-    W.nwpr = nwpr;
-    W.dw = dw;
-    W.roi = roi;
-    W.sw = a.sheds + (size_t)w * S;
-    W.delta = kDeltaLds ? delta : nullptr;
 
-    // the owner marks w used in its state
-    const int w_r = W.wx / a.cs, w_cc = W.wy / a.cs;
-    const int w_u = w_r * Q + w_cc % Q;
-    if (w_u % G == g) {
-      const int ul = w_u / G, k = w_cc / Q;
-      const int lo = OFF(ul, k), hi = OFF(ul, k + 1);
-      for (int i = lo + tid; i < hi; i += kLoopThreads)
-        if (C[i] == w) U[i] = 1;
+    // ---- bookkeeping in pick order; the 95% rule may cut the round short
+    bool stop = false;
+    int kc = k;  // picks whose update lists the sequential loop evaluates
+    for (int i = 0; i < k; i++) {
+      const int w = (int)(~(uint32_t)bw_key[i]);
+      nused++;
+      cumarea += (long long)(bw_key[i] >> 32);
+      if (g == 0 && tid == 0) {
+        a.used[w] = 1;
+        a.order[nused - 1] = w;
+      }
+      if (100.0 * (double)cumarea / a.npts > 95.0) {  // cpp:601-604
+        stop = true;
+        k = i + 1;
+        kc = i;
+        break;
+      }
+      niter++;
     }
-    if (kDeltaLds)
+    // owners mark the picks used
+    for (int i = 0; i < k; i++) {
+      const int w = (int)(~(uint32_t)bw_key[i]);
+      const int wx = (int)(bw_pos[i] >> 16), wy = (int)(bw_pos[i] & 0xffffu);
+      const int w_r = wx / a.cs, w_cc = wy / a.cs;
+      const int w_u = w_r * Q + w_cc % Q;
+      if (w_u % G == g) {
+        const int ul = w_u / G, kc2 = w_cc / Q;
+        const int lo = OFF(ul, kc2), hi = OFF(ul, kc2 + 1);
+        for (int j = lo + tid; j < hi; j += kLoopThreads)
+          if (C[j] == w) U[j] = 1;
+        if (tid == 0) dirty[ul] = 1;
+      }
+    }
+
+    // ---- union of the picks (cpp:393-419): task (pick, row), rows of one
+    // pick are disjoint, different picks may share an edge word (atomic OR)
+    Winner W;
+    {
+      const int w = (int)(~(uint32_t)bw_key[0]);
+      W.w = w;
+      W.wx = (int)(bw_pos[0] >> 16);
+      W.wy = (int)(bw_pos[0] & 0xffffu);
+      col_split(W.wy - roi, W.w0w, W.fbw);
+      W.nr = nr;
+      W.nwpr = nwpr;
+      W.dw = dw;
+      W.roi = roi;
+      W.sw = a.sheds + (size_t)w * S;
+      W.delta = nullptr;
+    }
+    const bool single = k == 1;
+    if (single && kDeltaLds) {
       for (int i = tid; i < nr * dw; i += kLoopThreads) {
         const int dr = i / dw, dk = i - (i / dw) * dw;
         delta[i] = winner_word(W.sw, nwpr, W.fbw, dr, dk);
       }
+      W.delta = delta;
+    }
     __syncthreads();
-
-    // ---- union: this workgroup's share of the winner's rows (cpp:393-419)
     {
-      const int my_rows = (nr - g + G - 1) / G;
-      for (int i = tid; i < my_rows * dw; i += kLoopThreads) {
-        const int t = i / dw, dk = i - (i / dw) * dw;
-        const int dr = g + t * G;
-        const int x = W.wx - roi + dr, j = W.w0w + dk;
-        if (x >= 0 && x < n && j >= 0 && j < cw) a.cum[(size_t)x * cw + j] |= W.word(dr, dk);
+      const int total = k * nr * dw;
+      const int chunk = (total + G - 1) / G;
+      const int t0 = g * chunk, t1 = min(total, t0 + chunk);
+      for (int t = t0 + tid; t < t1; t += kLoopThreads) {
+        const int i = t / (nr * dw), rem = t - i * (nr * dw);
+        const int dr = rem / dw, dk = rem - (rem / dw) * dw;
+        const int wx = (int)(bw_pos[i] >> 16), wy = (int)(bw_pos[i] & 0xffffu);
+        const int x = wx - roi + dr;
+        int w0w, fbw;
+        col_split(wy - roi, w0w, fbw);
+        const int j = w0w + dk;
+        if (x < 0 || x >= n || j < 0 || j >= cw) continue;
+        const uint64_t *sw = a.sheds + (size_t)(int)(~(uint32_t)bw_key[i]) * S;
+        const uint64_t v = winner_word(sw, nwpr, fbw, dr, dk);
+        if (v == 0) continue;
+        if (single)
+          a.cum[(size_t)x * cw + j] |= v;
+        else
+          atomicOr((unsigned long long *)&a.cum[(size_t)x * cw + j], (unsigned long long)v);
       }
     }
-    if (100.0 * (double)cumarea / a.npts > 95.0) break;  // cpp:601-604
-    niter++;
+    // several picks: the update lists read cum after every pick's union
+    if (!single) grid_sync(&a.st->barrier, bar_target, (unsigned)G);
 
-    // ---- update candidates of the owned units near the winner
-    const int crlo = max(0, (W.wx - 2 * roi) / a.cs);
-    const int crhi = min(a.ncr - 1, (W.wx + 2 * roi) / a.cs);
-    const int cclo = max(0, (W.wy - 2 * roi) / a.cs);
-    const int cchi = min(a.ncr - 1, (W.wy + 2 * roi) / a.cs);
+    // ---- update lists of the picks 0..kc-1 within the owned units
     for (int ul = 0; ul < upw; ul++) {
       const int u = ul * G + g;
+      if (u >= nunits) break;
       const int r = u / Q;
-      const bool touched = u < a.nunits && r >= crlo && r <= crhi;
-      if (tid == 0) dirty[ul] = touched;
+      bool touched = false;
+      for (int i = 0; i < kc; i++) {
+        const int wx = (int)(bw_pos[i] >> 16);
+        if (r >= max(0, (wx - 2 * roi) / a.cs) && r <= (wx + 2 * roi) / a.cs) touched = true;
+      }
       if (!touched) continue;
-      const int q = u - r * Q;
-      const int klo = cclo <= q ? 0 : (cclo - q + Q - 1) / Q;
-      const int khi = cchi < q ? -1 : (cchi - q) / Q;
-      if (klo > khi) continue;
-      const int ulo = OFF(ul, klo), uhi = OFF(ul, khi + 1);
-      for (int base = ulo; base < uhi; base += kMaxList) {
-        const int span = min(kMaxList, uhi - base);
-        if (tid < span) {  // update list, cpp:540-544
+      const int lo = OFF(ul, 0), hi = OFF(ul, kpu);
+      for (int base = lo; base < hi; base += kMaxList) {
+        const int span = min(kMaxList, hi - base);
+        if (tid < span) {  // cpp:540-544, once per pick whose list holds it
           const int i = base + tid;
-          const uint32_t ps = P[i];
-          const int cx = (int)(ps >> 16), cy = (int)(ps & 0xffffu);
-          const int ex = cx - W.wx, ey = cy - W.wy;
-          if (!U[i] && C[i] != w && ex * ex + ey * ey <= r2) {
-            int acc = 0;
-            if (a.intervis) {  // cpp:452-457: own cell covered by cum'?
-              uint64_t word = a.cum[(size_t)cx * cw + (cy >> 6)];
-              const int dr = cx - W.wx + roi, dk = (cy >> 6) - W.w0w;
-              if (dr >= 0 && dr < nr && dk >= 0 && dk <= nwpr) word |= W.word(dr, dk);
-              if (!((word >> (63 - (cy & 63))) & 1u)) acc = -1;
+          if (!U[i]) {
+            const uint32_t ps = P[i];
+            const int cx = (int)(ps >> 16), cy = (int)(ps & 0xffffu);
+            int m = 0;
+            for (int j = 0; j < kc; j++) {
+              const int ex = cx - (int)(bw_pos[j] >> 16);
+              const int ey = cy - (int)(bw_pos[j] & 0xffffu);
+              m += ex * ex + ey * ey <= r2;
             }
-            const int at = atomicAdd(&list_n, 1);
-            list_i[at] = i;
-            list_acc[at] = acc;
+            if (m) {
+              int acc = m;
+              if (!stop && a.intervis) {  // cpp:452-457 (kmax == 1 here)
+                uint64_t word = a.cum[(size_t)cx * cw + (cy >> 6)];
+                const int dr = cx - W.wx + roi, dk = (cy >> 6) - W.w0w;
+                if (dr >= 0 && dr < nr && dk >= 0 && dk <= nwpr) word |= W.word(dr, dk);
+                if (!((word >> (63 - (cy & 63))) & 1u)) acc = -m;
+              }
+              const int at = atomicAdd(&list_n, 1);
+              list_i[at] = i;
+              list_acc[at] = acc;
+            }
           }
         }
         __syncthreads();
         const int lcnt = list_n;
-        my_u += lcnt;
-        // one wavefront per update candidate, lanes over consecutive words
-        for (int e = wid; e < lcnt; e += kLoopWaves) {
-          if (list_acc[e] < 0) continue;
-          const int i = list_i[e];
-          const uint32_t ps = P[i];
-          const int s = wave_new_bits(a.sheds + (size_t)C[i] * S, (int)(ps >> 16),
-                                      (int)(ps & 0xffffu), a, W, lane);
-          if (lane == 0) list_acc[e] = s;
+        for (int e = tid; e < lcnt; e += kLoopThreads) {
+          const int v = list_acc[e];
+          my_u += v < 0 ? -v : v;
+        }
+        __syncthreads();  // counts read before the areas overwrite them
+        if (!stop) {
+          // one wavefront per update candidate, lanes over consecutive words
+          for (int e = wid; e < lcnt; e += kLoopWaves) {
+            if (list_acc[e] < 0) continue;
+            const int i = list_i[e];
+            const uint32_t ps = P[i];
+            const int s = wave_new_bits(a.sheds + (size_t)C[i] * S, (int)(ps >> 16),
+                                        (int)(ps & 0xffffu), a, W, lane, single);
+            if (lane == 0) list_acc[e] = s;
+          }
+          __syncthreads();
+          if (tid < lcnt) {
+            const int acc = list_acc[tid];
+            T[list_i[tid]] = acc < 0 ? 0 : acc;
+          }
         }
         __syncthreads();
-        if (tid < lcnt) {
-          const int acc = list_acc[tid];
-          T[list_i[tid]] = acc < 0 ? 0 : acc;
+        if (tid == 0) {
+          list_n = 0;
+          dirty[ul] = 1;
         }
-        __syncthreads();
-        if (tid == 0) list_n = 0;
         __syncthreads();
       }
     }
-    // the winner's unit changed too (w is used now)
-    if (w_u % G == g && tid == 0) dirty[w_u / G] = 1;
+    if (stop) break;
     __syncthreads();
     for (int ul = wid; ul < upw; ul += kLoopWaves)
-      if (dirty[ul]) unit_key(ul, w);
+      if (dirty[ul]) unit_top2(ul);
     __syncthreads();
-    uint64_t *kn = a.keys + (size_t)(parity ^ 1) * a.nunits;
-    uint32_t *pn = a.kpos + (size_t)(parity ^ 1) * a.nunits;
-    for (int ul = tid; ul < upw; ul += kLoopThreads) {
-      const int u = ul * G + g;
-      if (u < a.nunits) {
-        kn[u] = ukey[ul];
-        pn[u] = upos[ul];
-      }
-    }
+    for (int ul = tid; ul < upw; ul += kLoopThreads) dirty[ul] = 0;
     parity ^= 1;
+    publish(parity);
     grid_sync(&a.st->barrier, bar_target, (unsigned)G);
   }
 
-  if (tid == 0 && my_u) atomicAdd(&a.st->sum_u, (unsigned long long)my_u);
+  // sum_u: each thread counted its own list entries
+  {
+    unsigned long long v = (unsigned long long)my_u;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    if (lane == 0 && v) atomicAdd(&a.st->sum_u, v);
+  }
   if (g == 0 && tid == 0) {
     a.st->nused = nused;
     a.st->niter = niter;
@@ -566,9 +673,10 @@ __global__ __launch_bounds__(kLoopThreads) void site_loop(SiteArgs a) {
 // ---- host-side plan --------------------------------------------------------
 
 struct Plan {
-  int cs, ncr, Q, nunits, G, upw, kpu, nbuckets, nchunks, cap, mw, off_lds;
+  int cs, ncr, Q, nunits, G, upw, kpu, nbuckets, nchunks, cap, delta_lds, off_lds;
   size_t lds;
-  int o_delta, o_ukey, o_upos, o_dirty, o_off, o_c, o_pos, o_t, o_used;
+  int o_delta, o_ukey, o_upos, o_dirty, o_off, o_c, o_pos, o_t, o_used, o_sk, o_sp,
+      o_stk;
 };
 
 int env_q() {
@@ -605,14 +713,21 @@ Plan make_plan(int n, int roi, int nsheds, int groups_cap) {
   p.nbuckets = p.G * p.upw * p.kpu;
   p.nchunks = (p.nbuckets + 1023) / 1024;
   const int nwpr = (2 * roi + 1 + 63) / 64;
-  p.mw = nwpr + 1 <= 3 ? 3 : (nwpr + 1 <= 5 ? 5 : (nwpr + 1 <= 8 ? 8 : 0));
+  const size_t delta_bytes = sizeof(uint64_t) * (size_t)(2 * roi + 1) * (nwpr + 1);
+  p.delta_lds = delta_bytes <= 40 * 1024;
   size_t o = 0;
   p.o_delta = (int)o;
-  if (p.mw) o += sizeof(uint64_t) * (size_t)(2 * roi + 1) * (nwpr + 1);
+  if (p.delta_lds) o += delta_bytes;
+  p.o_sk = (int)o;
+  o += sizeof(uint64_t) * 2 * (size_t)p.nunits;
   p.o_ukey = (int)o;
-  o += sizeof(uint64_t) * p.upw;
+  o += sizeof(uint64_t) * 2 * (size_t)p.upw;
+  p.o_sp = (int)o;
+  o += sizeof(uint32_t) * 2 * (size_t)p.nunits;
   p.o_upos = (int)o;
-  o += sizeof(uint32_t) * p.upw;
+  o += sizeof(uint32_t) * 2 * (size_t)p.upw;
+  p.o_stk = (int)o;
+  o += (size_t)p.nunits;
   p.o_dirty = (int)o;
   o = align_up(o + (size_t)p.upw, 16);
   p.o_off = (int)o;
@@ -663,8 +778,8 @@ size_t scratch_layout(const Plan &p, int nsheds, uint8_t *base, Scratch *out) {
   s.spos = (uint32_t *)take(4 * (size_t)nsheds);
   s.sT = (int32_t *)take(4 * (size_t)nsheds);
   s.sused = (uint8_t *)take((size_t)nsheds);
-  s.keys = (uint64_t *)take(8 * 2 * (size_t)p.nunits);
-  s.kpos = (uint32_t *)take(4 * 2 * (size_t)p.nunits);
+  s.keys = (uint64_t *)take(8 * 4 * (size_t)p.nunits);
+  s.kpos = (uint32_t *)take(4 * 4 * (size_t)p.nunits);
   if (out) *out = s;
   return o;
 }
@@ -733,6 +848,9 @@ siting_status launch_site(const Geometry &g, const int32_t *obs,
   a.o_ukey = p.o_ukey;
   a.o_upos = p.o_upos;
   a.o_dirty = p.o_dirty;
+  a.o_sk = p.o_sk;
+  a.o_sp = p.o_sp;
+  a.o_stk = p.o_stk;
   a.o_off = p.o_off;
   a.off_lds = p.off_lds;
   a.o_c = p.o_c;
@@ -750,20 +868,10 @@ siting_status launch_site(const Geometry &g, const int32_t *obs,
   hipLaunchKernelGGL(site_bucket_scatter, dim3(nbk), dim3(tb), 0, s, a);
   hipLaunchKernelGGL(site_first_pass, dim3((nsheds + 3) / 4), dim3(256), 0, s, a);
   SITING_HIP(hipGetLastError());
-  switch (p.mw) {
-    case 3:
-      hipLaunchKernelGGL(site_loop<3>, dim3(p.G), dim3(kLoopThreads), p.lds, s, a);
-      break;
-    case 5:
-      hipLaunchKernelGGL(site_loop<5>, dim3(p.G), dim3(kLoopThreads), p.lds, s, a);
-      break;
-    case 8:
-      hipLaunchKernelGGL(site_loop<8>, dim3(p.G), dim3(kLoopThreads), p.lds, s, a);
-      break;
-    default:
-      hipLaunchKernelGGL(site_loop<0>, dim3(p.G), dim3(kLoopThreads), p.lds, s, a);
-      break;
-  }
+  if (p.delta_lds)
+    hipLaunchKernelGGL(site_loop<true>, dim3(p.G), dim3(kLoopThreads), p.lds, s, a);
+  else
+    hipLaunchKernelGGL(site_loop<false>, dim3(p.G), dim3(kLoopThreads), p.lds, s, a);
   SITING_HIP(hipGetLastError());
   return SITING_OK;
 }
