@@ -68,6 +68,7 @@ struct SiteArgs {
   uint32_t *spos;    // [nsheds] (x << 16 | y), bucket order
   int32_t *sT;       // [nsheds] area state of workgroups in global mode
   uint8_t *sused;    // [nsheds] used state of workgroups in global mode
+  uint64_t *nb;      // [kMaxBatch][nr][nwpr + 1] bits each pick newly covered
   int n, roi, nr, nwpr, cw, nsheds, intervis;
   int cs, ncr, Q, nunits, G, upw, kpu, nbuckets, cap;
   // dynamic LDS layout (byte offsets)
@@ -294,6 +295,66 @@ __device__ __forceinline__ int wave_new_bits(const uint64_t *sh, int cx, int cy,
   return wave_sum_i32(s);
 }
 
+// Bits of candidate (cx, cy)'s shed that this round's picks newly covered,
+// i.e. the drop of its area, summed over the wavefront.  Only picks within
+// 2*roi share bits with it.  Items are (row, terrain word) pairs of the
+// overlap of the two boxes; a word the pick did not change is skipped
+// without loading the candidate's shed.
+__device__ __forceinline__ int wave_lost_bits(const uint64_t *sh, int cx, int cy,
+                                              const SiteArgs &a, int k,
+                                              const uint32_t *bw_pos, int lane) {
+  const int roi = a.roi, nr = a.nr, nwpr = a.nwpr, cw = a.cw, n = a.n;
+  const int dw = nwpr + 1;
+  const int r2 = 4 * roi * roi;
+  int w0c, fbc;
+  col_split(cy - roi, w0c, fbc);
+  const int wlc = (cy + roi) >> 6;
+  int s = 0;
+  for (int p = 0; p < k; p++) {
+    const int wx = (int)(bw_pos[p] >> 16), wy = (int)(bw_pos[p] & 0xffffu);
+    const int ex = cx - wx, ey = cy - wy;
+    if (ex * ex + ey * ey > r2) continue;
+    int w0p, fbp;
+    col_split(wy - roi, w0p, fbp);
+    const int wlp = (wy + roi) >> 6;
+    const int xlo = max(max(cx, wx) - roi, 0), xhi = min(min(cx, wx) + roi, n - 1);
+    const int jlo = max(max(w0c, w0p), 0), jhi = min(min(wlc, wlp), cw - 1);
+    if (xlo > xhi || jlo > jhi) continue;
+    const int nk = jhi - jlo + 1;
+    const int items = (xhi - xlo + 1) * nk;  // <= nr * (nwpr + 1)
+    const float inv_nk = 1.0f / (float)nk;
+    const bool fdiv = items < (1 << 21);
+    const uint64_t *nbp = a.nb + (size_t)p * nr * dw;
+    for (int base = 0; base < items; base += kWave * kBatch) {
+      uint64_t nbv[kBatch], cur[kBatch], prv[kBatch];
+      int soff[kBatch], kk[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; u++) {
+        const int it = base + u * kWave + lane;
+        const bool ok = it < items;
+        const int q = fdiv ? (int)(((float)it + 0.5f) * inv_nk) : it / nk;
+        const int x = xlo + q, j = jlo + (it - q * nk);
+        nbv[u] = ok ? nbp[(size_t)(x - wx + roi) * dw + (j - w0p)] : 0;
+        soff[u] = (x - cx + roi) * nwpr;
+        kk[u] = j - w0c;
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; u++) {
+        const bool live = nbv[u] != 0;
+        cur[u] = (live && kk[u] < nwpr) ? sh[soff[u] + kk[u]] : 0;
+        prv[u] = (live && kk[u] > 0) ? sh[soff[u] + kk[u] - 1] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; u++) {
+        uint64_t moved = cur[u] >> fbc;
+        if (fbc != 0) moved |= prv[u] << (64 - fbc);
+        s += __popcll(moved & nbv[u]);
+      }
+    }
+  }
+  return wave_sum_i32(s);
+}
+
 // Best and second best (key, position) over the unused candidates of owned
 // unit ul; one wavefront.
 struct Top2 {
@@ -440,18 +501,27 @@ __global__ __launch_bounds__(kLoopThreads) void site_loop(SiteArgs a) {
     }
     __syncthreads();
     if (wid == 0) {
-      int k = 0;
-      while (k < kmax) {
-        uint64_t bv = 0;
-        int bu = -1;
+      // lane-local best over units lane, lane+64, ...: the unit's first key,
+      // or its second once the first is taken this round (or the taken
+      // second, as an upper bound of the unknown third)
+      auto lane_best = [&](uint64_t &lv, int &lu) {
+        lv = 0;
+        lu = -1;
         for (int u = lane; u < nunits; u += kWave) {
-          const int t = stk[u];
-          const uint64_t v = sk[2 * u + (t == 0 ? 0 : 1)];
-          if (v > bv) {
-            bv = v;
-            bu = u;
+          const uint64_t v = sk[2 * u + (stk[u] == 0 ? 0 : 1)];
+          if (v > lv) {
+            lv = v;
+            lu = u;
           }
         }
+      };
+      uint64_t lv;
+      int lu;
+      lane_best(lv, lu);
+      int k = 0;
+      while (k < kmax) {
+        uint64_t bv = lv;
+        int bu = lu;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
           const uint64_t ov = __shfl_xor(bv, o, kWave);
@@ -476,10 +546,12 @@ __global__ __launch_bounds__(kLoopThreads) void site_loop(SiteArgs a) {
         if (lane == 0) {
           bw_key[k] = bv;
           bw_pos[k] = pos;
+        }
+        if ((bu & (kWave - 1)) == lane) {  // the lane that holds unit bu
           stk[bu] = (uint8_t)(t + 1);
+          lane_best(lv, lu);
         }
         k++;
-        __builtin_amdgcn_wave_barrier();
       }
       if (lane == 0) bw_n = k;
     }
@@ -525,8 +597,7 @@ __global__ __launch_bounds__(kLoopThreads) void site_loop(SiteArgs a) {
       }
     }
 
-    // ---- union of the picks (cpp:393-419): task (pick, row), rows of one
-    // pick are disjoint, different picks may share an edge word (atomic OR)
+    // ---- union of the picks into cum (cpp:393-419)
     Winner W;
     {
       const int w = (int)(~(uint32_t)bw_key[0]);
@@ -541,16 +612,32 @@ __global__ __launch_bounds__(kLoopThreads) void site_loop(SiteArgs a) {
       W.sw = a.sheds + (size_t)w * S;
       W.delta = nullptr;
     }
-    const bool single = k == 1;
-    if (single && kDeltaLds) {
-      for (int i = tid; i < nr * dw; i += kLoopThreads) {
-        const int dr = i / dw, dk = i - (i / dw) * dw;
-        delta[i] = winner_word(W.sw, nwpr, W.fbw, dr, dk);
+    if (a.intervis) {
+      // one pick: areas are recomputed against cum | delta while the union
+      // is written (a reader sees either word, the OR is the same)
+      if (kDeltaLds) {
+        for (int i = tid; i < nr * dw; i += kLoopThreads) {
+          const int dr = i / dw, dk = i - (i / dw) * dw;
+          delta[i] = winner_word(W.sw, nwpr, W.fbw, dr, dk);
+        }
+        W.delta = delta;
       }
-      W.delta = delta;
-    }
-    __syncthreads();
-    {
+      __syncthreads();
+      const int total = nr * dw;
+      const int chunk = (total + G - 1) / G;
+      const int t0 = g * chunk, t1 = min(total, t0 + chunk);
+      for (int t = t0 + tid; t < t1; t += kLoopThreads) {
+        const int dr = t / dw, dk = t - (t / dw) * dw;
+        const int x = W.wx - roi + dr, j = W.w0w + dk;
+        if (x < 0 || x >= n || j < 0 || j >= cw) continue;
+        const uint64_t v = W.word(dr, dk);
+        if (v) a.cum[(size_t)x * cw + j] |= v;
+      }
+    } else {
+      // every pick: OR its shifted shed into cum and keep the bits that were
+      // new (the pick's area; picks are > 2*roi apart so their sheds share
+      // no bit, only, possibly, an edge word -- hence the atomic)
+      __syncthreads();
       const int total = k * nr * dw;
       const int chunk = (total + G - 1) / G;
       const int t0 = g * chunk, t1 = min(total, t0 + chunk);
@@ -562,88 +649,80 @@ __global__ __launch_bounds__(kLoopThreads) void site_loop(SiteArgs a) {
         int w0w, fbw;
         col_split(wy - roi, w0w, fbw);
         const int j = w0w + dk;
-        if (x < 0 || x >= n || j < 0 || j >= cw) continue;
-        const uint64_t *sw = a.sheds + (size_t)(int)(~(uint32_t)bw_key[i]) * S;
-        const uint64_t v = winner_word(sw, nwpr, fbw, dr, dk);
-        if (v == 0) continue;
-        if (single)
-          a.cum[(size_t)x * cw + j] |= v;
-        else
-          atomicOr((unsigned long long *)&a.cum[(size_t)x * cw + j], (unsigned long long)v);
+        uint64_t nbv = 0;
+        if (x >= 0 && x < n && j >= 0 && j < cw) {
+          const uint64_t *sw = a.sheds + (size_t)(int)(~(uint32_t)bw_key[i]) * S;
+          const uint64_t v = winner_word(sw, nwpr, fbw, dr, dk);
+          if (v) {
+            const uint64_t old = atomicOr((unsigned long long *)&a.cum[(size_t)x * cw + j],
+                                          (unsigned long long)v);
+            nbv = v & ~old;
+          }
+        }
+        a.nb[t] = nbv;
       }
+      grid_sync(&a.st->barrier, bar_target, (unsigned)G);
     }
-    // several picks: the update lists read cum after every pick's union
-    if (!single) grid_sync(&a.st->barrier, bar_target, (unsigned)G);
 
-    // ---- update lists of the picks 0..kc-1 within the owned units
-    for (int ul = 0; ul < upw; ul++) {
-      const int u = ul * G + g;
-      if (u >= nunits) break;
-      const int r = u / Q;
-      bool touched = false;
-      for (int i = 0; i < kc; i++) {
-        const int wx = (int)(bw_pos[i] >> 16);
-        if (r >= max(0, (wx - 2 * roi) / a.cs) && r <= (wx + 2 * roi) / a.cs) touched = true;
-      }
-      if (!touched) continue;
-      const int lo = OFF(ul, 0), hi = OFF(ul, kpu);
-      for (int base = lo; base < hi; base += kMaxList) {
-        const int span = min(kMaxList, hi - base);
-        if (tid < span) {  // cpp:540-544, once per pick whose list holds it
-          const int i = base + tid;
-          if (!U[i]) {
-            const uint32_t ps = P[i];
-            const int cx = (int)(ps >> 16), cy = (int)(ps & 0xffffu);
-            int m = 0;
-            for (int j = 0; j < kc; j++) {
-              const int ex = cx - (int)(bw_pos[j] >> 16);
-              const int ey = cy - (int)(bw_pos[j] & 0xffffu);
-              m += ex * ex + ey * ey <= r2;
+    // ---- update lists of picks 0..kc-1 among this workgroup's candidates
+    for (int base = 0; base < cnt; base += kMaxList) {
+      if (tid < min(kMaxList, cnt - base)) {  // cpp:540-544, once per list
+        const int i = base + tid;
+        if (!U[i]) {
+          const uint32_t ps = P[i];
+          const int cx = (int)(ps >> 16), cy = (int)(ps & 0xffffu);
+          int m = 0;
+          for (int j = 0; j < kc; j++) {
+            const int ex = cx - (int)(bw_pos[j] >> 16);
+            const int ey = cy - (int)(bw_pos[j] & 0xffffu);
+            m += ex * ex + ey * ey <= r2;
+          }
+          if (m) {
+            int acc = m;
+            if (!stop && a.intervis) {  // cpp:452-457 (one pick here)
+              uint64_t word = a.cum[(size_t)cx * cw + (cy >> 6)];
+              const int dr = cx - W.wx + roi, dk = (cy >> 6) - W.w0w;
+              if (dr >= 0 && dr < nr && dk >= 0 && dk <= nwpr) word |= W.word(dr, dk);
+              if (!((word >> (63 - (cy & 63))) & 1u)) acc = -m;
             }
-            if (m) {
-              int acc = m;
-              if (!stop && a.intervis) {  // cpp:452-457 (kmax == 1 here)
-                uint64_t word = a.cum[(size_t)cx * cw + (cy >> 6)];
-                const int dr = cx - W.wx + roi, dk = (cy >> 6) - W.w0w;
-                if (dr >= 0 && dr < nr && dk >= 0 && dk <= nwpr) word |= W.word(dr, dk);
-                if (!((word >> (63 - (cy & 63))) & 1u)) acc = -m;
-              }
-              const int at = atomicAdd(&list_n, 1);
-              list_i[at] = i;
-              list_acc[at] = acc;
-            }
+            const int at = atomicAdd(&list_n, 1);
+            list_i[at] = i;
+            list_acc[at] = acc;
+            dirty[((cx / a.cs) * Q + (cy / a.cs) % Q) / G] = 1;
           }
         }
-        __syncthreads();
-        const int lcnt = list_n;
-        for (int e = tid; e < lcnt; e += kLoopThreads) {
-          const int v = list_acc[e];
-          my_u += v < 0 ? -v : v;
-        }
-        __syncthreads();  // counts read before the areas overwrite them
-        if (!stop) {
-          // one wavefront per update candidate, lanes over consecutive words
-          for (int e = wid; e < lcnt; e += kLoopWaves) {
-            if (list_acc[e] < 0) continue;
-            const int i = list_i[e];
-            const uint32_t ps = P[i];
-            const int s = wave_new_bits(a.sheds + (size_t)C[i] * S, (int)(ps >> 16),
-                                        (int)(ps & 0xffffu), a, W, lane, single);
-            if (lane == 0) list_acc[e] = s;
-          }
-          __syncthreads();
-          if (tid < lcnt) {
-            const int acc = list_acc[tid];
-            T[list_i[tid]] = acc < 0 ? 0 : acc;
-          }
-        }
-        __syncthreads();
-        if (tid == 0) {
-          list_n = 0;
-          dirty[ul] = 1;
-        }
-        __syncthreads();
       }
+      __syncthreads();
+      const int lcnt = list_n;
+      for (int e = tid; e < lcnt; e += kLoopThreads) {
+        const int v = list_acc[e];
+        my_u += v < 0 ? -v : v;
+      }
+      __syncthreads();  // counts read before the areas overwrite them
+      if (!stop && lcnt > 0) {
+        // one wavefront per update candidate
+        for (int e = wid; e < lcnt; e += kLoopWaves) {
+          if (list_acc[e] < 0) continue;
+          const int i = list_i[e];
+          const uint32_t ps = P[i];
+          const int cx = (int)(ps >> 16), cy = (int)(ps & 0xffffu);
+          const uint64_t *sh = a.sheds + (size_t)C[i] * S;
+          int s;
+          if (a.intervis)
+            s = wave_new_bits(sh, cx, cy, a, W, lane, true);
+          else  // exact area minus the bits this round's picks covered
+            s = T[i] - wave_lost_bits(sh, cx, cy, a, k, bw_pos, lane);
+          if (lane == 0) list_acc[e] = s;
+        }
+        __syncthreads();
+        if (tid < lcnt) {
+          const int acc = list_acc[tid];
+          T[list_i[tid]] = acc < 0 ? 0 : acc;
+        }
+      }
+      __syncthreads();
+      if (tid == 0) list_n = 0;
+      __syncthreads();
     }
     if (stop) break;
     __syncthreads();
@@ -674,7 +753,7 @@ __global__ __launch_bounds__(kLoopThreads) void site_loop(SiteArgs a) {
 
 struct Plan {
   int cs, ncr, Q, nunits, G, upw, kpu, nbuckets, nchunks, cap, delta_lds, off_lds;
-  size_t lds;
+  size_t lds, nbwords;
   int o_delta, o_ukey, o_upos, o_dirty, o_off, o_c, o_pos, o_t, o_used, o_sk, o_sp,
       o_stk;
 };
@@ -714,6 +793,7 @@ Plan make_plan(int n, int roi, int nsheds, int groups_cap) {
   p.nchunks = (p.nbuckets + 1023) / 1024;
   const int nwpr = (2 * roi + 1 + 63) / 64;
   const size_t delta_bytes = sizeof(uint64_t) * (size_t)(2 * roi + 1) * (nwpr + 1);
+  p.nbwords = (size_t)kMaxBatch * (2 * roi + 1) * (nwpr + 1);
   p.delta_lds = delta_bytes <= 40 * 1024;
   size_t o = 0;
   p.o_delta = (int)o;
@@ -758,6 +838,7 @@ struct Scratch {
   uint8_t *sused;
   uint64_t *keys;
   uint32_t *kpos;
+  uint64_t *nb;
 };
 
 size_t scratch_layout(const Plan &p, int nsheds, uint8_t *base, Scratch *out) {
@@ -778,6 +859,7 @@ size_t scratch_layout(const Plan &p, int nsheds, uint8_t *base, Scratch *out) {
   s.spos = (uint32_t *)take(4 * (size_t)nsheds);
   s.sT = (int32_t *)take(4 * (size_t)nsheds);
   s.sused = (uint8_t *)take((size_t)nsheds);
+  s.nb = (uint64_t *)take(8 * p.nbwords);
   s.keys = (uint64_t *)take(8 * 4 * (size_t)p.nunits);
   s.kpos = (uint32_t *)take(4 * 4 * (size_t)p.nunits);
   if (out) *out = s;
@@ -828,6 +910,7 @@ siting_status launch_site(const Geometry &g, const int32_t *obs,
   a.spos = sc.spos;
   a.sT = sc.sT;
   a.sused = sc.sused;
+  a.nb = sc.nb;
   a.n = n;
   a.roi = roi;
   a.nr = g.d.nr;
