@@ -74,6 +74,7 @@ struct siting_job {
   SiteBuffers site{};
   float *recip = nullptr;
   uint32_t *jump = nullptr;
+  double *rdiv = nullptr;
   hipEvent_t ev_start[4] = {};
   hipEvent_t ev_end[4] = {};
   bool ran[4] = {};
@@ -83,7 +84,7 @@ namespace {
 
 struct Layout {
   size_t elevs, vix, obs, sheds, area, used, cum, rowmax, order, state, recip,
-      jump, scratch, total;
+      jump, rdiv, scratch, total;
 };
 
 size_t round_up(size_t v) { return (v + 255) / 256 * 256; }
@@ -109,6 +110,7 @@ Layout layout_for(const siting_params &p, const siting_derived &d) {
   put(L.state, sizeof(SiteState));
   put(L.recip, ((size_t)p.roi + 1) * 4);
   put(L.jump, 4 * (size_t)p.ntests * 4);
+  put(L.rdiv, ((size_t)p.roi + 1) * 8);
   put(L.scratch, site_scratch_bytes(p.nrows, p.roi, d.nwanted));
   L.total = off;
   return L;
@@ -224,13 +226,18 @@ siting_status siting_job_create(int device, const siting_params *p, siting_job *
   j->site.scratch = b + L.scratch;
   j->recip = (float *)(b + L.recip);
   j->jump = (uint32_t *)(b + L.jump);
+  j->rdiv = (double *)(b + L.rdiv);
   std::vector<float> rec((size_t)p->roi + 1);
   std::vector<uint32_t> jmp(4 * (size_t)p->ntests);
+  std::vector<double> rdv((size_t)p->roi + 1);
   fill_recip_table(p->roi, rec.data());
   fill_jump_table(p->ntests, jmp.data());
+  fill_rdiv_table(p->roi, rdv.data());
   e = hipMemcpy(j->recip, rec.data(), rec.size() * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(j->jump, jmp.data(), jmp.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(j->rdiv, rdv.data(), rdv.size() * 8, hipMemcpyHostToDevice);
   if (e != hipSuccess) return fail(hip_fail(e, "hipMemcpy(tables)", __FILE__, __LINE__));
   *out = j;
   return SITING_OK;
@@ -279,7 +286,7 @@ siting_status siting_job_findmax(siting_job *j) {
   JOB_STAGE(1, launch_findmax(j->g, j->vix, j->obs, j->stream));
 }
 siting_status siting_job_viewshed(siting_job *j) {
-  JOB_STAGE(2, launch_viewshed(j->g, j->elevs, j->obs, j->sheds, j->stream));
+  JOB_STAGE(2, launch_viewshed(j->g, j->elevs, j->obs, j->sheds, j->rdiv, j->stream));
 }
 siting_status siting_job_site(siting_job *j) {
   JOB_STAGE(3, launch_site(j->g, j->obs, j->sheds, j->site, j->device, j->stream));
@@ -489,14 +496,18 @@ siting_status siting_calc_sheds(int nrows, const uint16_t *elevs, int roi, int o
   g.d.nr = 2 * roi + 1;
   g.d.nwpr = (g.d.nr + 63) / 64;
   const size_t sbytes = (size_t)nsheds * g.d.nr * g.d.nwpr * 8;
-  DevBuf de, dobs, dsh;
+  DevBuf de, dobs, dsh, drd;
   CHECK(dalloc(de, n2 * 2));
   CHECK(dalloc(dobs, (size_t)nsheds * 4));
   CHECK(dalloc(dsh, sbytes));
+  CHECK(dalloc(drd, ((size_t)roi + 1) * 8));
+  std::vector<double> rdv((size_t)roi + 1);
+  fill_rdiv_table(roi, rdv.data());
   SITING_HIP(hipMemcpyAsync(de.p, elevs, n2 * 2, hipMemcpyHostToDevice, s));
   SITING_HIP(hipMemcpyAsync(dobs.p, obs, (size_t)nsheds * 4, hipMemcpyHostToDevice, s));
+  SITING_HIP(hipMemcpyAsync(drd.p, rdv.data(), rdv.size() * 8, hipMemcpyHostToDevice, s));
   CHECK(launch_viewshed(g, (const uint16_t *)de.p, (const int32_t *)dobs.p,
-                        (uint64_t *)dsh.p, s));
+                        (uint64_t *)dsh.p, (const double *)drd.p, s));
   SITING_HIP(hipMemcpyAsync(sheds, dsh.p, sbytes, hipMemcpyDeviceToHost, s));
   SITING_HIP(hipStreamSynchronize(s));
   return SITING_OK;

@@ -116,9 +116,10 @@ siting_status launch_vix(const Geometry &g, const uint16_t *elevs, uint8_t *vix,
                          const VixTables &t, hipStream_t s);
 siting_status launch_findmax(const Geometry &g, uint8_t *vix, int32_t *obs,
                              hipStream_t s);
+// rdiv: 1.0/a in double for a = 0..roi (entry 0 unused)
 siting_status launch_viewshed(const Geometry &g, const uint16_t *elevs,
                               const int32_t *obs, uint64_t *sheds,
-                              hipStream_t s);
+                              const double *rdiv, hipStream_t s);
 
 struct SiteBuffers {
   int32_t *area;      // [nwanted] testshedarea
@@ -142,5 +143,6 @@ siting_status launch_site(const Geometry &g, const int32_t *obs,
 // host tables for VIX
 void fill_recip_table(int roi, float *out);               // roi+1 entries
 void fill_jump_table(int ntests, uint32_t *out);          // 4*ntests entries
+void fill_rdiv_table(int roi, double *out);               // roi+1 entries
 
 }  // namespace siting

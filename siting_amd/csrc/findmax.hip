@@ -79,6 +79,71 @@ __global__ __launch_bounds__(kWave *kWavesPerGroup) void findmax_kernel(
   }
 }
 
+// Blocks of at most 64*PER cells: every lane keeps the (vix, hash) keys of
+// its cells in registers, read once; a round's winner is zeroed in the
+// owning lane's copy (and with a byte store in memory), so later rounds
+// never re-read vix.
+template <int PER>
+__global__ __launch_bounds__(kWave *kWavesPerGroup) void findmax_regs_kernel(
+    uint8_t *__restrict__ vix, int32_t *__restrict__ obs, int n, float bs,
+    int nb, int nwpb) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int blk = blockIdx.x * kWavesPerGroup + (threadIdx.x >> 6);
+  if (blk >= nb * nb) return;  // whole wavefronts exit together
+  const int bx = blk / nb, by = blk - (blk / nb) * nb;
+  const int x0 = (int)(bs * (float)bx);
+  const int x1 = min((int)(bs * (float)(bx + 1)), n);
+  const int y0 = (int)(bs * (float)by);
+  const int y1 = min((int)(bs * (float)(by + 1)), n);
+  const int width = y1 - y0;
+  const int np = (x1 - x0) * width;
+  if (np <= 0) return;  // derive() admits no empty block; kept as a guard
+
+  uint64_t key[PER];  // vix << 32 | hash ^ 0x80000000; cell q = lane + 64*t
+#pragma unroll
+  for (int t = 0; t < PER; t++) {
+    const int q = lane + kWave * t;
+    key[t] = 0;
+    if (q < np) {
+      const int x = x0 + q / width, y = y0 + q % width;
+      key[t] = (uint64_t)vix[(size_t)x * n + y] << 32 |
+               (uint32_t)((uint32_t)block_hash(x, y) ^ 0x80000000u);
+    }
+  }
+  for (int round = 0; round < nwpb; round++) {
+    uint64_t hi = 0;
+    int j = 0x7fffffff;  // cells past np keep j > np, so any real cell wins
+#pragma unroll
+    for (int t = 0; t < PER; t++) {
+      const int q = lane + kWave * t;
+      if (q < np && (key[t] > hi || (key[t] == hi && q < j))) {
+        hi = key[t];
+        j = q;
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const uint64_t oh = __shfl_xor(hi, off, kWave);
+      const int oj = __shfl_xor(j, off, kWave);
+      if (oh > hi || (oh == hi && oj < j)) {
+        hi = oh;
+        j = oj;
+      }
+    }
+    if ((j & (kWave - 1)) == lane) {  // owner: this cell's vix becomes 0
+#pragma unroll
+      for (int t = 0; t < PER; t++)
+        if (lane + kWave * t == j) key[t] &= 0xffffffffull;
+    }
+    if (lane == 0) {
+      const int x = x0 + j / width, y = y0 + j % width;
+      const size_t idx = (size_t)x * n + y;
+      obs[(size_t)blk * nwpb + round] = (int32_t)idx;
+      vix[idx] = 0;  // siting.cpp:231
+    }
+  }
+}
+
 }  // namespace
 
 siting_status launch_findmax(const Geometry &g, uint8_t *vix, int32_t *obs,
@@ -86,8 +151,19 @@ siting_status launch_findmax(const Geometry &g, uint8_t *vix, int32_t *obs,
   const int nb = g.d.nblocks;
   const int nblk = nb * nb;
   const int groups = (nblk + kWavesPerGroup - 1) / kWavesPerGroup;
-  hipLaunchKernelGGL(findmax_kernel, dim3(groups), dim3(kWave * kWavesPerGroup),
-                     0, s, vix, obs, g.p.nrows, g.d.blocksize, nb, g.d.nwpb);
+  // largest block: edge (int)(bs*(b+1)) - (int)(bs*b) <= ceil(bs) + 1
+  const int edge = (int)g.d.blocksize + 2;
+  const long long maxcells = (long long)edge * edge;
+  const dim3 grid(groups), block(kWave * kWavesPerGroup);
+  if (maxcells <= 2 * kWave)
+    hipLaunchKernelGGL(findmax_regs_kernel<2>, grid, block, 0, s, vix, obs, g.p.nrows,
+                       g.d.blocksize, nb, g.d.nwpb);
+  else if (maxcells <= 8 * kWave)
+    hipLaunchKernelGGL(findmax_regs_kernel<8>, grid, block, 0, s, vix, obs, g.p.nrows,
+                       g.d.blocksize, nb, g.d.nwpb);
+  else
+    hipLaunchKernelGGL(findmax_kernel, grid, block, 0, s, vix, obs, g.p.nrows,
+                       g.d.blocksize, nb, g.d.nwpb);
   SITING_HIP(hipGetLastError());
   return SITING_OK;
 }

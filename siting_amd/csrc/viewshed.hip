@@ -6,9 +6,10 @@
 // of the box perimeter (each corner once, siting.cpp:292-304); adjacent lanes
 // take adjacent perimeter cells, so at every step the wavefront reads a
 // short arc of neighbouring DEM cells.  Along a ray the running horizon
-// slope is kept in float exactly as the reference computes it: IEEE
-// divisions for the ray slope and the point slope, round-half-away for the
-// minor coordinate, no fused multiply-add (siting.cpp:323-344).  Visible
+// slope is kept in float exactly as the reference computes it: the IEEE
+// quotient for the ray slope and the point slope (the latter via an exact
+// double reciprocal, see below), round-half-away for the minor coordinate,
+// no fused multiply-add (siting.cpp:323-344).  Visible
 // cells are OR'ed into the shed bitset held in LDS (MSB-first 64-bit words,
 // siting.cpp:247-250); the finished bitset goes to HBM with coalesced
 // 64-bit stores, zero words included, so no separate clear is needed.  When
@@ -18,30 +19,37 @@
 // Observers are assigned XCD-major (workgroup b runs on XCD b%8, each XCD
 // takes a contiguous run of observers, which FINDMAX emitted block by block,
 // i.e. spatially adjacent), so each XCD's L2 serves one band of the DEM.
+#include <cstdlib>
+
 #include "common.hpp"
 
 namespace siting {
 
 namespace {
 
-constexpr int kThreads = 256;
+constexpr int kMaxThreads = 1024;
 constexpr int kLdsWords = 8192;  // 64 KiB bitset budget
 
-template <bool kInLds>
-__global__ __launch_bounds__(kThreads) void viewshed_kernel(
+template <bool kInLds, bool kRdLds>
+__global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
     const uint16_t *__restrict__ elevs, const int32_t *__restrict__ obs,
-    uint64_t *__restrict__ sheds, int n, int roi, int oht, int tht, int nwpr,
-    int nsheds, int per_xcd) {
-  extern __shared__ uint64_t lds_bits[];
+    uint64_t *__restrict__ sheds, const double *__restrict__ rdiv, int n, int roi,
+    int oht, int tht, int nwpr, int nsheds, int per_xcd) {
+  const int nthreads = blockDim.x;
+  extern __shared__ uint64_t lds_dyn[];  // [roi + 1] reciprocals?, bitset?
   const int b = blockIdx.x;
   const int k = (b & 7) * per_xcd + (b >> 3);
   if (k >= nsheds) return;
   const int nr = 2 * roi + 1;
   const int words = nr * nwpr;
   uint64_t *out = sheds + (size_t)k * words;
-  uint64_t *bits = kInLds ? lds_bits : out;
+  double *rd_l = (double *)lds_dyn;
+  const double *rd = kRdLds ? rd_l : rdiv;
+  uint64_t *bits = kInLds ? lds_dyn + (kRdLds ? roi + 1 : 0) : out;
 
-  for (int i = threadIdx.x; i < words; i += kThreads) bits[i] = 0;
+  if (kRdLds)
+    for (int i = threadIdx.x; i <= roi; i += nthreads) rd_l[i] = rdiv[i];
+  for (int i = threadIdx.x; i < words; i += nthreads) bits[i] = 0;
   if (!kInLds) __threadfence();
   __syncthreads();
 
@@ -52,15 +60,11 @@ __global__ __launch_bounds__(kThreads) void viewshed_kernel(
   const int rr = roi * roi;
   const int twor = 2 * roi;
 
-  if (threadIdx.x == 0) {  // observer sees itself, siting.cpp:279
-    const uint64_t m = 1ull << (63 - (roi & 63));
-    if (kInLds)
-      atomicOr((unsigned long long *)&bits[roi * nwpr + (roi >> 6)], m);
-    else
-      atomicOr((unsigned long long *)&bits[roi * nwpr + (roi >> 6)], m);
-  }
+  if (threadIdx.x == 0)  // observer sees itself, siting.cpp:279
+    atomicOr((unsigned long long *)&bits[roi * nwpr + (roi >> 6)],
+             1ull << (63 - (roi & 63)));
 
-  for (int ray = threadIdx.x; ray < 8 * roi; ray += kThreads) {
+  for (int ray = threadIdx.x; ray < 8 * roi; ray += nthreads) {
     const int side = ray / twor;
     const int j = ray - side * twor;
     int dx, dy;
@@ -95,7 +99,12 @@ __global__ __launch_bounds__(kThreads) void viewshed_kernel(
       if (qx * qx + qy * qy > rr) break;                  // siting.cpp:334
       const int pel = (int)elevs[px * n + py];
       const float fa = (float)a;
-      const float sl = (float)(pel - oalt) / fa;  // siting.cpp:340
+      // siting.cpp:340, (float)(pel - oalt) / (float)a.  With |pel - oalt| <
+      // 2^21 and a < 2^14 the quotient is never closer than 2^-39 (relative)
+      // to a float rounding midpoint, and never on one, while the double
+      // product below is within 2^-52 of it: rounding it to float gives the
+      // correctly rounded fp32 quotient, bit for bit.
+      const float sl = (float)((double)(pel - oalt) * rd[a]);
       if (horizon < sl) horizon = sl;
       const float rise = horizon * fa;
       const float halt = foalt + rise;  // siting.cpp:342
@@ -109,29 +118,44 @@ __global__ __launch_bounds__(kThreads) void viewshed_kernel(
 
   if (kInLds) {
     __syncthreads();
-    for (int i = threadIdx.x; i < words; i += kThreads) out[i] = bits[i];
+    for (int i = threadIdx.x; i < words; i += nthreads) out[i] = bits[i];
   }
 }
 
 }  // namespace
 
+void fill_rdiv_table(int roi, double *out) {
+  out[0] = 0.0;
+  for (int a = 1; a <= roi; a++) out[a] = 1.0 / (double)a;
+}
+
 siting_status launch_viewshed(const Geometry &g, const uint16_t *elevs,
                               const int32_t *obs, uint64_t *sheds,
-                              hipStream_t s) {
+                              const double *rdiv, hipStream_t s) {
   const int nsheds = g.d.nwanted;
   if (nsheds <= 0) return SITING_OK;
   const int per_xcd = (nsheds + 7) / 8;
   const int words = g.d.nr * g.d.nwpr;
-  if (words <= kLdsWords) {
-    hipLaunchKernelGGL(viewshed_kernel<true>, dim3(8 * per_xcd), dim3(kThreads),
-                       sizeof(uint64_t) * words, s, elevs, obs, sheds,
-                       g.p.nrows, g.p.roi, g.p.oht, g.p.tht, g.d.nwpr, nsheds,
-                       per_xcd);
-  } else {
-    hipLaunchKernelGGL(viewshed_kernel<false>, dim3(8 * per_xcd), dim3(kThreads),
-                       0, s, elevs, obs, sheds, g.p.nrows, g.p.roi, g.p.oht,
-                       g.p.tht, g.d.nwpr, nsheds, per_xcd);
+  // 256 threads (4 wavefronts) per observer, rays dealt round-robin; the
+  // count can be overridden for tuning
+  int threads = 256;
+  if (const char *e = std::getenv("SITING_VIEWSHED_THREADS")) {
+    const int t = std::atoi(e);
+    if (t >= kWave && t <= kMaxThreads && t % kWave == 0) threads = t;
   }
+  const bool rd_lds = g.p.roi < 4096;  // 32 KiB table
+  const size_t rd_bytes = rd_lds ? sizeof(double) * ((size_t)g.p.roi + 1) : 0;
+  const bool in_lds = words <= kLdsWords;
+  const size_t lds = rd_bytes + (in_lds ? sizeof(uint64_t) * words : 0);
+#define SITING_VS(A, B)                                                             \
+  hipLaunchKernelGGL((viewshed_kernel<A, B>), dim3(8 * per_xcd), dim3(threads), lds, \
+                     s, elevs, obs, sheds, rdiv, g.p.nrows, g.p.roi, g.p.oht, g.p.tht, \
+                     g.d.nwpr, nsheds, per_xcd)
+  if (in_lds && rd_lds) SITING_VS(true, true);
+  else if (in_lds) SITING_VS(true, false);
+  else if (rd_lds) SITING_VS(false, true);
+  else SITING_VS(false, false);
+#undef SITING_VS
   SITING_HIP(hipGetLastError());
   return SITING_OK;
 }
