@@ -1,20 +1,28 @@
 """Siting throughput benchmark on one MI355X.
 
-A step is one pass of the four-stage pipeline (VIX -> FINDMAX -> VIEWSHED ->
-SITE) over the resident DEM of the plain single-job workload: a 4096x4096
-uint16 fractal+hills terrain (H 0.8, heights 0..20000, 8 Gaussian hills of
-peak 5000 and sigma 80 cells, seed 2), roi 100, height 100, 64 targets,
-bw 16, nwanted 65536, intervis 0.  The DEM is in HBM before the timed region
-starts; every stage output stays in HBM.
+Two workloads, both on synthetic seeded terrain (project generator:
+diamond-square roughness H, linear rescale to 0..hmax, 8 Gaussian hills of
+peak 5000 and sigma 80 cells), DEMs in HBM before any timed region:
 
-Prints one JSON line.  Besides throughput it reports, per stage, the HIP
-event time measured on the job's stream and the stage's algorithmic bytes,
-a roofline entry for the dominant stage, a bit-exactness check of the GPU
-result against the CPU oracle, and the oracle's OpenMP variant timed on this
-host's cores on the same workload (cpu_baseline).
+  uneven batch  64 independent jobs: edge log-uniform in [512, 4096], H in
+                [0.5, 0.9], heights 0..20000, roi in {25,50,100,150},
+                targets in {32,64}, bw in {8,16}, nwanted = n^2/256,
+                height 100, intervis 0.  A step is the whole batch through
+                VIX -> FINDMAX -> VIEWSHED -> SITE.  Timed batched (one
+                batch object: stream pool, one work-queue SITE launch, HIP
+                graph replay) and plain (the jobs one by one, each synced).
+  single job    4096^2, H 0.8, roi 100, height 100, 64 targets, bw 16,
+                nwanted 65536, intervis 0: per-stage HIP-event times,
+                algorithmic bytes and the roofline entry of the dominant
+                stage; bit-exactness against the CPU oracle; the oracle's
+                OpenMP variant timed on this host's cores (cpu_baseline).
+
+Prints one JSON line; `value` is the batched throughput (terrain points per
+second over the whole batch).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -23,9 +31,36 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = ("terrain pts sited/sec batched vs plain GPU (bit-exact); SITE HBM GB/s; "
+          "p99 job lat")
 
-WORKLOAD = dict(nrows=4096, roi=100, height=100, ntests=64, bw=16, nwanted0=65536,
-                intervis=0, rough_h=0.8, hmax=20000, seed=2)
+SINGLE = dict(nrows=4096, roi=100, height=100, ntests=64, bw=16, nwanted0=65536,
+              intervis=0, rough_h=0.8, hmax=20000, seed=2)
+
+
+def uneven_batch(njobs=64, seed=2024):
+    """The uneven batch: one seeded draw per job."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for j in range(njobs):
+        n = int(round(math.exp(rng.uniform(math.log(512), math.log(4096)))))
+        jobs.append(dict(nrows=n, rough_h=float(rng.uniform(0.5, 0.9)), hmax=20000,
+                         roi=int(rng.choice([25, 50, 100, 150])),
+                         ntests=int(rng.choice([32, 64])), bw=int(rng.choice([8, 16])),
+                         nwanted0=n * n // 256, height=100, intervis=0, seed=5000 + j))
+    return jobs
+
+
+def make_dem(sa, w):
+    return sa.dem_generate(sa.TerrainSpec(nrows=w["nrows"], rough_h=w["rough_h"], hmin=0,
+                                          hmax=w["hmax"], nhills=8, hill_peak=5000.0,
+                                          hill_sigma=80.0, seed=w["seed"]))
+
+
+def params_of(sa, w):
+    return sa.make_params(w["nrows"], w["roi"], w["height"], w["ntests"], w["bw"],
+                          w["nwanted0"], w["intervis"])
 
 
 def algorithmic_bytes(n, d, st):
@@ -40,22 +75,19 @@ def algorithmic_bytes(n, d, st):
     }
 
 
-def cpu_baseline(dem, w, threads):
-    import oracle
-    t0 = time.perf_counter()
-    ref = oracle.run(w["nrows"], dem, w["roi"], w["height"], w["ntests"], w["bw"],
-                     w["nwanted0"], w["intervis"], nthreads=threads)
-    dt = time.perf_counter() - t0
-    return ref, dt
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-graph", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the "
@@ -66,7 +98,58 @@ def main():
     import numpy as np
     import siting_amd as sa
 
-    w = WORKLOAD
+    # ---------------- uneven batch: batched vs plain ----------------------
+    batch = uneven_batch(args.jobs)
+    dems = [make_dem(sa, w) for w in batch]
+    pts = sum(w["nrows"] ** 2 for w in batch)
+    log(f"uneven batch: {len(batch)} jobs, {pts} points, DEMs generated")
+    params = [params_of(sa, w) for w in batch]
+    b = sa.Batch(params, use_graph=not args.no_graph)
+    for i, dem in enumerate(dems):
+        b.upload_dem(i, dem)
+    for i in range(args.warmup):
+        b.run()
+        b.sync()
+        log(f"batch warmup {i} done")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.run()
+    b.sync()
+    t_batch = (time.perf_counter() - t0) / args.steps
+    dev_batch_ms = b.elapsed_ms()
+    log(f"batched: {1000 * t_batch:.1f} ms/step (device {dev_batch_ms:.1f} ms last step)")
+    batch_sel = [b.job(i).download_selected() for i in range(len(batch))]
+    batch_cum = [b.job(i).stats().cumarea for i in range(len(batch))]
+    b.close()
+
+    # plain: the same jobs one at a time, each stage synced as the CLI does
+    jobs = []
+    for i, w in enumerate(batch):
+        j = sa.Job(params[i])
+        j.upload_dem(dems[i])
+        jobs.append(j)
+    for j in jobs[:4]:  # warm the code paths
+        j.run()
+        j.sync()
+    t0 = time.perf_counter()
+    for j in jobs:
+        j.vix()
+        j.sync()
+        j.findmax()
+        j.sync()
+        j.viewshed()
+        j.sync()
+        j.site()
+        j.sync()
+    t_plain = time.perf_counter() - t0
+    same = all(np.array_equal(jobs[i].download_selected(), batch_sel[i]) and
+               jobs[i].stats().cumarea == batch_cum[i] for i in range(len(jobs)))
+    for j in jobs:
+        j.close()
+    log(f"plain one-by-one: {1000 * t_plain:.1f} ms; batch==plain: {same}")
+
+    # ---------------- single job: stages, roofline, oracle ------------------
+    w = SINGLE
     n = w["nrows"]
     dem = sa.dem_generate(sa.TerrainSpec(nrows=n, rough_h=w["rough_h"], hmin=0,
                                          hmax=w["hmax"], nhills=8, hill_peak=5000.0,
@@ -76,34 +159,21 @@ def main():
     job = sa.Job(p)
     job.upload_dem(dem)
     d = job.derived
-    print(f"[bench] workload n={n} nwanted={d.nwanted} job bytes={job.bytes_needed(p)}",
-          file=sys.stderr, flush=True)
-
-    for i in range(args.warmup):
-        job.run()
-        job.sync()
-        print(f"[bench] warmup {i} done", file=sys.stderr, flush=True)
-
-    stage_ms = {"vix": 0.0, "findmax": 0.0, "viewshed": 0.0, "site": 0.0}
+    job.run()
     job.sync()
+    stage_ms = {"vix": 0.0, "findmax": 0.0, "viewshed": 0.0, "site": 0.0}
     t0 = time.perf_counter()
     for _ in range(args.steps):
         job.run()
-        job.sync()  # per-step sync so every step's stage events are read
+        job.sync()
         st = job.stats()
         stage_ms["vix"] += st.ms_vix
         stage_ms["findmax"] += st.ms_findmax
         stage_ms["viewshed"] += st.ms_viewshed
         stage_ms["site"] += st.ms_site
-    job.sync()
-    elapsed = time.perf_counter() - t0
+    t_single = (time.perf_counter() - t0) / args.steps
     for k in stage_ms:
         stage_ms[k] /= args.steps
-    ms_per_step = 1000.0 * elapsed / args.steps
-    value = n * n * args.steps / elapsed
-    print(f"[bench] timed {args.steps} steps: {ms_per_step:.2f} ms/step", file=sys.stderr,
-          flush=True)
-
     sel = job.download_selected()
     obs = job.download_obs()
     abytes = algorithmic_bytes(n, d, st)
@@ -114,49 +184,66 @@ def main():
                      "gb_s": round(gbs, 2) if gbs else None}
     dom = max(stage_ms, key=stage_ms.get)
     ach = abytes[dom] / (stage_ms[dom] * 1e-3) / 1e9
+    site_gbs = abytes["site"] / (stage_ms["site"] * 1e-3) / 1e9
     roofline = {"stage": dom, "bound": "hbm", "achieved": round(ach, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
                 "traffic": None}
+    log(f"single job: {1000 * t_single:.2f} ms/step, stages {stage_ms}")
 
+    value = pts / t_batch
     result = {
-        "metric": "terrain points sited per second (plain single-job GPU path, "
-                  "bit-exact vs sequential reference)",
+        "metric": METRIC,
         "value": round(value, 1),
         "unit": "points/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(1000 * t_batch, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+u64",
         "data": "synthetic",
-        "config": {"workload": "plain single-job: 4096^2 uint16 fractal+hills DEM, "
-                               "roi 100, height 100, targets 64, bw 16, nwanted 65536, "
-                               "intervis 0",
-                   "nwanted": d.nwanted, "shed_bytes": 8 * d.shed_words},
-        "site": {"nused": st.nused, "niter": st.niter, "sum_u": st.sum_u,
-                 "coverage": st.coverage},
-        "stages": stages,
+        "config": {"workload": f"uneven batch of {len(batch)} siting jobs (edge 512..4096 "
+                               "log-uniform, roi 25..150, targets 32/64, bw 8/16, "
+                               "nwanted n^2/256, intervis 0)",
+                   "points": pts,
+                   "batched_points_per_s": round(value, 1),
+                   "plain_one_by_one_points_per_s": round(pts / t_plain, 1),
+                   "batched_over_plain": round(t_plain / t_batch, 3),
+                   "batch_equals_plain": same,
+                   "graph": not args.no_graph},
+        "single_job": {"workload": "4096^2 fractal+hills DEM, roi 100, height 100, "
+                                   "targets 64, bw 16, nwanted 65536, intervis 0",
+                       "points_per_s": round(n * n / t_single, 1),
+                       "ms_per_step": round(1000 * t_single, 3),
+                       "stages": stages,
+                       "site_hbm_gb_s": round(site_gbs, 2),
+                       "site": {"nused": st.nused, "niter": st.niter, "sum_u": st.sum_u,
+                                "coverage": st.coverage}},
         "roofline": roofline,
         "cpu_baseline": None,
     }
 
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        print(f"[bench] cpu baseline with {threads} threads", file=sys.stderr, flush=True)
-        ref, dt = cpu_baseline(dem, w, threads)
+        log(f"cpu baseline with {threads} threads")
+        import oracle
+        t0 = time.perf_counter()
+        ref = oracle.run(n, dem, w["roi"], w["height"], w["ntests"], w["bw"],
+                         w["nwanted0"], w["intervis"], nthreads=threads)
+        dt = time.perf_counter() - t0
         parity = bool(np.array_equal(sel, ref["selected"]) and
                       np.array_equal(obs, ref["obs"]) and
                       st.cumarea == ref["stats"].cumarea)
-        result["parity_vs_oracle"] = parity
+        result["single_job"]["parity_vs_oracle"] = parity
         result["cpu_baseline"] = {
             "value": round(n * n / dt, 1), "unit": "points/s", "cores": threads,
             "kind": "port",
-            "sample": "the full workload once (all four stages), OpenMP variant of "
-                      "the C restatement of the sequential reference",
+            "sample": "the single-job workload once (4096^2, all four stages), OpenMP "
+                      "variant of the C restatement of the sequential reference",
             "seconds": round(dt, 3)}
+    job.close()
     print(json.dumps(result), flush=True)
 
 

@@ -175,6 +175,36 @@ siting_status siting_job_device_ptr(siting_job *job, siting_buffer which,
 siting_status siting_job_stats(siting_job *job, siting_stats *st);
 siting_status siting_job_derived(siting_job *job, siting_derived *d);
 
+/* ---- batched jobs -------------------------------------------------------
+ * Many independent jobs (any mix of sizes and parameters) run as one unit:
+ * the per-job VIX, FINDMAX, VIEWSHED and SITE-preparation kernels go to a
+ * small pool of streams so they overlap, and every job's greedy SITE loop
+ * runs in a single persistent work-queue launch whose slots of workgroups
+ * take jobs, largest first, as they free up.  The launch sequence can be
+ * captured once into a HIP graph and replayed.  Results are bit-identical
+ * to running each job alone. */
+
+typedef struct siting_batch siting_batch;
+
+typedef struct {
+  int streams;   /* stream pool size (0: 4) */
+  int slot;      /* workgroups per SITE slot (0: tuned default) */
+  int nslots;    /* SITE slots (0: CUs / slot) */
+  int use_graph; /* 1: capture the run into a HIP graph on first use */
+} siting_batch_opts;
+
+siting_status siting_batch_create(int device, int njobs, const siting_params *params,
+                                  const siting_batch_opts *opts, siting_batch **out);
+void siting_batch_destroy(siting_batch *b);
+siting_status siting_batch_upload_dem(siting_batch *b, int job, const uint16_t *elevs);
+/* Enqueue all stages of all jobs (asynchronous). */
+siting_status siting_batch_run(siting_batch *b);
+siting_status siting_batch_sync(siting_batch *b);
+/* Device time of the last run, from events around it (after sync). */
+siting_status siting_batch_elapsed_ms(siting_batch *b, float *ms);
+/* Borrowed handle of job i, for downloads and stats (valid until destroy). */
+siting_status siting_batch_job(siting_batch *b, int job, siting_job **out);
+
 /* ---- terrain input (raw uint16 format, siting.cpp:726-730) ------------ */
 
 /* Exactly nrows*nrows native-endian uint16 values; SITING_EIO if short. */

@@ -111,6 +111,13 @@ _SIGS = {
     "siting_dem_generate": (_I, [_I, ctypes.c_double, _I, _I, _I, ctypes.c_double,
                                  ctypes.c_double, ctypes.c_uint64, _VP]),
     "siting_device_count": (_I, []),
+    "siting_batch_create": (_I, [_I, _I, _VP, _VP, ctypes.POINTER(_VP)]),
+    "siting_batch_destroy": (None, [_VP]),
+    "siting_batch_upload_dem": (_I, [_VP, _I, _VP]),
+    "siting_batch_run": (_I, [_VP]),
+    "siting_batch_sync": (_I, [_VP]),
+    "siting_batch_elapsed_ms": (_I, [_VP, ctypes.POINTER(ctypes.c_float)]),
+    "siting_batch_job": (_I, [_VP, _I, ctypes.POINTER(_VP)]),
 }
 
 
@@ -305,6 +312,79 @@ class Job:
 
     def download_order(self, nused: int):
         return self._download(BUF_ORDER, np.int32, nused)
+
+
+# ---- batches ---------------------------------------------------------------
+
+class BatchOpts(ctypes.Structure):
+    _fields_ = [("streams", ctypes.c_int), ("slot", ctypes.c_int),
+                ("nslots", ctypes.c_int), ("use_graph", ctypes.c_int)]
+
+
+class _JobView(Job):
+    """A job owned by a batch (not destroyed by this handle)."""
+
+    def __init__(self, handle, params):
+        self.params = params
+        self._h = handle
+        d = Derived()
+        _check(lib().siting_job_derived(self._h, ctypes.byref(d)))
+        self.derived = d
+
+    def close(self):
+        self._h = None
+
+
+class Batch:
+    """Many independent jobs run as one unit (see siting.h, batched jobs)."""
+
+    def __init__(self, params_list, device=0, streams=0, slot=0, nslots=0, use_graph=False):
+        arr = (Params * len(params_list))(*params_list)
+        self.params = list(params_list)
+        opts = BatchOpts(streams, slot, nslots, 1 if use_graph else 0)
+        h = ctypes.c_void_p()
+        _check(lib().siting_batch_create(device, len(params_list), ctypes.cast(arr, _VP),
+                                         ctypes.byref(opts), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().siting_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload_dem(self, i, elevs):
+        e = np.ascontiguousarray(elevs, dtype=np.uint16).reshape(-1)
+        if e.size != self.params[i].nrows ** 2:
+            raise SitingError(EINVAL, "DEM size mismatch")
+        _check(lib().siting_batch_upload_dem(self._h, i, _ptr(e)))
+
+    def run(self):
+        _check(lib().siting_batch_run(self._h))
+
+    def sync(self):
+        _check(lib().siting_batch_sync(self._h))
+
+    def elapsed_ms(self) -> float:
+        ms = ctypes.c_float()
+        _check(lib().siting_batch_elapsed_ms(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def job(self, i) -> Job:
+        h = ctypes.c_void_p()
+        _check(lib().siting_batch_job(self._h, i, ctypes.byref(h)))
+        return _JobView(h, self.params[i])
 
 
 # ---- terrain ---------------------------------------------------------------

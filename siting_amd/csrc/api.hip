@@ -3,6 +3,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
@@ -564,6 +565,203 @@ siting_status siting_site_it(int nrows, int roi, int intervis, int nsheds,
     stats->sum_u = (int64_t)st.sum_u;
     stats->coverage = 100.0 * (double)st.cumarea / (double)n2;
   }
+  return SITING_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+struct siting_batch {
+  int device = 0;
+  std::vector<siting_job *> jobs;
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> joins;
+  hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
+  std::vector<int> order;          // jobs by descending SITE work
+  SiteArgs *args_dev = nullptr;    // [njobs] in queue order
+  unsigned int *ctl = nullptr;     // work-queue counters
+  size_t lds = 0;
+  int slot = 16, nslots = 1;
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+siting_status batch_enqueue(siting_batch *b) {
+  hipStream_t s0 = b->streams[0];
+  SITING_HIP(hipEventRecord(b->fork, s0));
+  for (size_t k = 1; k < b->streams.size(); k++)
+    SITING_HIP(hipStreamWaitEvent(b->streams[k], b->fork, 0));
+  for (size_t q = 0; q < b->order.size(); q++) {
+    siting_job *j = b->jobs[(size_t)b->order[q]];
+    hipStream_t s = b->streams[q % b->streams.size()];
+    siting_status st;
+    if ((st = launch_vix(j->g, j->elevs, j->vix, VixTables{j->recip, j->jump}, s)) != SITING_OK)
+      return st;
+    if ((st = launch_findmax(j->g, j->vix, j->obs, s)) != SITING_OK) return st;
+    if ((st = launch_viewshed(j->g, j->elevs, j->obs, j->sheds, j->rdiv, s)) != SITING_OK)
+      return st;
+    SiteArgs a;
+    if ((st = site_make_args(j->g, j->obs, j->sheds, j->site, b->slot, &a, nullptr)) !=
+        SITING_OK)
+      return st;
+    if ((st = site_launch_prep(a, s)) != SITING_OK) return st;
+  }
+  for (size_t k = 1; k < b->streams.size(); k++) {
+    SITING_HIP(hipEventRecord(b->joins[k], b->streams[k]));
+    SITING_HIP(hipStreamWaitEvent(s0, b->joins[k], 0));
+  }
+  return site_launch_queue(b->args_dev, (int)b->jobs.size(), b->slot, b->nslots, b->lds,
+                           b->ctl, s0);
+}
+
+}  // namespace
+
+extern "C" {
+
+void siting_batch_destroy(siting_batch *b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  for (hipStream_t s : b->streams)
+    if (s) (void)hipStreamSynchronize(s);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (hipEvent_t e : b->joins)
+    if (e) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->t0) (void)hipEventDestroy(b->t0);
+  if (b->t1) (void)hipEventDestroy(b->t1);
+  for (hipStream_t s : b->streams)
+    if (s) (void)hipStreamDestroy(s);
+  if (b->args_dev) (void)hipFree(b->args_dev);
+  if (b->ctl) (void)hipFree(b->ctl);
+  for (siting_job *j : b->jobs) siting_job_destroy(j);
+  delete b;
+}
+
+siting_status siting_batch_create(int device, int njobs, const siting_params *params,
+                                  const siting_batch_opts *opts, siting_batch **out) {
+  if (!out || !params || njobs < 1) return SITING_EINVAL;
+  *out = nullptr;
+  siting_batch *b = new siting_batch();
+  b->device = device;
+  auto fail = [&](siting_status s) {
+    siting_batch_destroy(b);
+    return s;
+  };
+  for (int i = 0; i < njobs; i++) {
+    siting_job *j = nullptr;
+    siting_status st = siting_job_create(device, &params[i], &j);
+    if (st != SITING_OK) return fail(st);
+    b->jobs.push_back(j);
+  }
+  int cus = 0;
+  SITING_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+  const int nstreams = opts && opts->streams > 0 ? opts->streams : 4;
+  b->slot = opts && opts->slot > 0 ? opts->slot : 16;
+  if (b->slot > cus) b->slot = cus;
+  b->nslots = opts && opts->nslots > 0 ? opts->nslots : cus / b->slot;
+  if (b->nslots * b->slot > cus) b->nslots = cus / b->slot;
+  if (b->nslots < 1) b->nslots = 1;
+  if (b->nslots > njobs) b->nslots = njobs;
+  b->use_graph = opts && opts->use_graph;
+  hipError_t e;
+  for (int k = 0; k < nstreams; k++) {
+    hipStream_t s = nullptr;
+    hipEvent_t ev = nullptr;
+    if ((e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess)
+      return fail(hip_fail(e, "hipStreamCreate", __FILE__, __LINE__));
+    b->streams.push_back(s);
+    if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess)
+      return fail(hip_fail(e, "hipEventCreate", __FILE__, __LINE__));
+    b->joins.push_back(ev);
+  }
+  if ((e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreate(&b->t0)) != hipSuccess || (e = hipEventCreate(&b->t1)) != hipSuccess)
+    return fail(hip_fail(e, "hipEventCreate", __FILE__, __LINE__));
+  // queue order: largest SITE work first (candidates x shed words)
+  b->order.resize((size_t)njobs);
+  for (int i = 0; i < njobs; i++) b->order[(size_t)i] = i;
+  std::vector<double> work((size_t)njobs);
+  for (int i = 0; i < njobs; i++)
+    work[(size_t)i] = (double)b->jobs[(size_t)i]->g.d.nwanted *
+                      (double)b->jobs[(size_t)i]->g.d.shed_words;
+  std::stable_sort(b->order.begin(), b->order.end(),
+                   [&](int x, int y) { return work[(size_t)x] > work[(size_t)y]; });
+  std::vector<SiteArgs> args((size_t)njobs);
+  for (int q = 0; q < njobs; q++) {
+    siting_job *j = b->jobs[(size_t)b->order[(size_t)q]];
+    size_t lds = 0;
+    siting_status st = site_make_args(j->g, j->obs, j->sheds, j->site, b->slot,
+                                      &args[(size_t)q], &lds);
+    if (st != SITING_OK) return fail(st);
+    b->lds = lds > b->lds ? lds : b->lds;
+    j->ran[3] = true;  // stats read the SITE state
+  }
+  if ((e = hipMalloc(&b->args_dev, sizeof(SiteArgs) * (size_t)njobs)) != hipSuccess ||
+      (e = hipMalloc(&b->ctl, 4 * site_queue_ctl_words(b->nslots))) != hipSuccess)
+    return fail(hip_fail(e, "hipMalloc(batch)", __FILE__, __LINE__));
+  if ((e = hipMemcpy(b->args_dev, args.data(), sizeof(SiteArgs) * (size_t)njobs,
+                     hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(hip_fail(e, "hipMemcpy(batch args)", __FILE__, __LINE__));
+  *out = b;
+  return SITING_OK;
+}
+
+siting_status siting_batch_upload_dem(siting_batch *b, int job, const uint16_t *elevs) {
+  if (!b || job < 0 || job >= (int)b->jobs.size()) return SITING_EINVAL;
+  return siting_job_upload_dem(b->jobs[(size_t)job], elevs);
+}
+
+siting_status siting_batch_run(siting_batch *b) {
+  if (!b) return SITING_EINVAL;
+  SITING_HIP(hipSetDevice(b->device));
+  hipStream_t s0 = b->streams[0];
+  SITING_HIP(hipEventRecord(b->t0, s0));
+  if (b->use_graph) {
+    if (!b->exec) {
+      SITING_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
+      siting_status st = batch_enqueue(b);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(s0, &g);
+      if (st != SITING_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+      }
+      if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+      b->graph = g;
+      SITING_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    SITING_HIP(hipGraphLaunch(b->exec, s0));
+  } else {
+    siting_status st = batch_enqueue(b);
+    if (st != SITING_OK) return st;
+  }
+  SITING_HIP(hipEventRecord(b->t1, s0));
+  return SITING_OK;
+}
+
+siting_status siting_batch_sync(siting_batch *b) {
+  if (!b) return SITING_EINVAL;
+  SITING_HIP(hipSetDevice(b->device));
+  for (hipStream_t s : b->streams) SITING_HIP(hipStreamSynchronize(s));
+  return SITING_OK;
+}
+
+siting_status siting_batch_elapsed_ms(siting_batch *b, float *ms) {
+  if (!b || !ms) return SITING_EINVAL;
+  SITING_HIP(hipEventSynchronize(b->t1));
+  SITING_HIP(hipEventElapsedTime(ms, b->t0, b->t1));
+  return SITING_OK;
+}
+
+siting_status siting_batch_job(siting_batch *b, int job, siting_job **out) {
+  if (!b || !out || job < 0 || job >= (int)b->jobs.size()) return SITING_EINVAL;
+  *out = b->jobs[(size_t)job];
   return SITING_OK;
 }
 

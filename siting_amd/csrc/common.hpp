@@ -131,6 +131,36 @@ struct SiteBuffers {
   uint8_t *scratch;   // site_scratch_bytes(): candidate buckets
 };
 
+// Everything one job's SITE kernels need (device pointers, geometry, plan).
+constexpr int kSiteMaxBatch = 16;  // picks per round (intervis 0)
+struct SiteArgs {
+  const int32_t *obs;
+  const uint64_t *sheds;
+  int32_t *area;     // [nsheds] first-pass area by candidate id
+  uint8_t *used;     // [nsheds] selected flags (the output mask)
+  uint64_t *cum;     // [n * cw]
+  uint64_t *keys;    // [2][nunits][2] best two keys per unit
+  uint32_t *kpos;    // [2][nunits][2] their positions
+  int32_t *order;    // [nsheds]
+  SiteState *st;
+  int32_t *bcount;   // [nbuckets] counts, then scatter cursors
+  int32_t *bstart;   // [nbuckets + 1]
+  int32_t *chunk;    // [nchunks] scan partial sums
+  int32_t *sid;      // [nsheds] candidate ids, bucket order
+  uint32_t *spos;    // [nsheds] (x << 16 | y), bucket order
+  int32_t *sT;       // [nsheds] area state of workgroups in global mode
+  uint8_t *sused;    // [nsheds] used state of workgroups in global mode
+  uint64_t *nb;      // [kSiteMaxBatch][nr][nwpr + 1] bits each pick newly covered
+  int n, roi, nr, nwpr, cw, nsheds, intervis;
+  int cs, ncr, Q, nunits, G, upw, kpu, nbuckets, cap;
+  // dynamic LDS layout (byte offsets)
+  int o_delta, o_ukey, o_upos, o_dirty, o_off, o_c, o_pos, o_t, o_used, o_sk, o_sp,
+      o_stk;
+  int off_lds;    // unit cell offsets kept in LDS (else read from bstart)
+  int delta_lds;  // the winner's shifted shed staged in LDS (else recomputed)
+  double npts;
+};
+
 // Device scratch SITE needs beyond SiteBuffers' arrays, and the size of the
 // double-buffered per-row key array (entries).
 size_t site_scratch_bytes(int nrows, int roi, int nsheds);
@@ -139,6 +169,19 @@ int site_key_rows(int nrows, int roi);
 siting_status launch_site(const Geometry &g, const int32_t *obs,
                           const uint64_t *sheds, const SiteBuffers &b,
                           int device, hipStream_t s);
+
+// Batched SITE: site_make_args plans one job for a work-queue slot of `slot`
+// workgroups (no launch); site_launch_prep enqueues its preparation kernels;
+// site_launch_queue runs every job's loop in one persistent launch of
+// nslots * slot workgroups.  ctl holds site_queue_ctl_words(nslots) words.
+siting_status site_make_args(const Geometry &g, const int32_t *obs,
+                             const uint64_t *sheds, const SiteBuffers &b, int slot,
+                             SiteArgs *out, size_t *lds);
+siting_status site_launch_prep(const SiteArgs &a, hipStream_t s);
+size_t site_queue_ctl_words(int nslots);
+siting_status site_launch_queue(const SiteArgs *jobs_dev, int njobs, int slot,
+                                int nslots, size_t lds, unsigned int *ctl,
+                                hipStream_t s);
 
 // host tables for VIX
 void fill_recip_table(int roi, float *out);               // roi+1 entries

@@ -49,34 +49,9 @@ constexpr int kLoopWaves = kLoopThreads / kWave;
 constexpr int kMaxList = 1024;
 constexpr size_t kLdsBudget = 140 * 1024;  // dynamic LDS of the loop kernel
 constexpr int kMaxGroups = 128;
-constexpr int kMaxBatch = 16;  // picks per round (intervis 0)
+constexpr int kMaxBatch = kSiteMaxBatch;
 
-struct SiteArgs {
-  const int32_t *obs;
-  const uint64_t *sheds;
-  int32_t *area;     // [nsheds] first-pass area by candidate id
-  uint8_t *used;     // [nsheds] selected flags (the output mask)
-  uint64_t *cum;     // [n * cw]
-  uint64_t *keys;    // [2][nunits][2] best two keys per unit
-  uint32_t *kpos;    // [2][nunits][2] their positions
-  int32_t *order;    // [nsheds]
-  SiteState *st;
-  int32_t *bcount;   // [nbuckets] counts, then scatter cursors
-  int32_t *bstart;   // [nbuckets + 1]
-  int32_t *chunk;    // [nchunks] scan partial sums
-  int32_t *sid;      // [nsheds] candidate ids, bucket order
-  uint32_t *spos;    // [nsheds] (x << 16 | y), bucket order
-  int32_t *sT;       // [nsheds] area state of workgroups in global mode
-  uint8_t *sused;    // [nsheds] used state of workgroups in global mode
-  uint64_t *nb;      // [kMaxBatch][nr][nwpr + 1] bits each pick newly covered
-  int n, roi, nr, nwpr, cw, nsheds, intervis;
-  int cs, ncr, Q, nunits, G, upw, kpu, nbuckets, cap;
-  // dynamic LDS layout (byte offsets)
-  int o_delta, o_ukey, o_upos, o_dirty, o_off, o_c, o_pos, o_t, o_used, o_sk, o_sp,
-      o_stk;
-  int off_lds;  // unit cell offsets kept in LDS (else read from bstart)
-  double npts;
-};
+
 
 __device__ __forceinline__ uint64_t cand_key(int area, int c) {
   return ((uint64_t)(uint32_t)area << 32) | (uint64_t)(~(uint32_t)c);
@@ -388,8 +363,8 @@ __device__ __forceinline__ Top2 wave_top2(Top2 t) {
   return t;
 }
 
-template <bool kDeltaLds>
-__global__ __launch_bounds__(kLoopThreads) void site_loop(SiteArgs a) {
+// The greedy loop of one job, run by workgroups g = 0..G-1 (all resident).
+__device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) {
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ int list_i[kMaxList];
   __shared__ int list_acc[kMaxList];
@@ -398,7 +373,7 @@ __global__ __launch_bounds__(kLoopThreads) void site_loop(SiteArgs a) {
   __shared__ uint32_t bw_pos[kMaxBatch];
   __shared__ int bw_n;
 
-  const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
+  const int tid = threadIdx.x;
   const int lane = tid & 63, wid = tid >> 6;
   const int n = a.n, roi = a.roi, nr = a.nr, nwpr = a.nwpr, cw = a.cw;
   const int Q = a.Q, upw = a.upw, kpu = a.kpu, nunits = a.nunits;
@@ -615,7 +590,7 @@ __global__ __launch_bounds__(kLoopThreads) void site_loop(SiteArgs a) {
     if (a.intervis) {
       // one pick: areas are recomputed against cum | delta while the union
       // is written (a reader sees either word, the OR is the same)
-      if (kDeltaLds) {
+      if (a.delta_lds) {
         for (int i = tid; i < nr * dw; i += kLoopThreads) {
           const int dr = i / dw, dk = i - (i / dw) * dw;
           delta[i] = winner_word(W.sw, nwpr, W.fbw, dr, dk);
@@ -749,6 +724,39 @@ __global__ __launch_bounds__(kLoopThreads) void site_loop(SiteArgs a) {
   }
 }
 
+__global__ __launch_bounds__(kLoopThreads) void site_loop(SiteArgs a) {
+  site_loop_body(a, blockIdx.x, gridDim.x);
+}
+
+// Work queue of jobs.  The grid is cut into slots of `slot` workgroups; the
+// first group of a slot takes the next job index with an atomic and the slot
+// runs that job's loop (a job planned for G < slot groups leaves the rest
+// idle), then takes another, until the queue is empty.  The job index is
+// double-buffered by the slot's turn so a late reader never sees the next
+// turn's value.
+__global__ __launch_bounds__(kLoopThreads) void site_loop_queue(
+    const SiteArgs *jobs, int njobs, int slot, unsigned int *queue,
+    unsigned int *slot_job, unsigned int *slot_bar) {
+  __shared__ int job;
+  const int sl = blockIdx.x / slot, g = blockIdx.x - sl * slot;
+  unsigned int target = 0;
+  for (int turn = 0;; turn++) {
+    unsigned int *cell = &slot_job[2 * sl + (turn & 1)];
+    if (g == 0 && threadIdx.x == 0)
+      __hip_atomic_store(cell, atomicAdd(queue, 1u), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    grid_sync(&slot_bar[sl], target, (unsigned)slot);
+    if (threadIdx.x == 0)
+      job = (int)__hip_atomic_load(cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int j = job;
+    if (j >= njobs) break;
+    const SiteArgs &a = jobs[j];
+    if (g < a.G) site_loop_body(a, g, a.G);
+    __syncthreads();
+  }
+}
+
 // ---- host-side plan --------------------------------------------------------
 
 struct Plan {
@@ -775,7 +783,7 @@ int cell_edge(int n, int roi) {
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-Plan make_plan(int n, int roi, int nsheds, int groups_cap) {
+Plan make_plan(int n, int roi, int nsheds, int groups_cap, int fixed_groups = 0) {
   Plan p{};
   p.cs = cell_edge(n, roi);
   p.ncr = (n + p.cs - 1) / p.cs;
@@ -783,7 +791,8 @@ Plan make_plan(int n, int roi, int nsheds, int groups_cap) {
   if (p.Q > p.ncr) p.Q = p.ncr;
   p.nunits = p.ncr * p.Q;
   const int touched = ((4 * roi) / p.cs + 2) * p.Q;
-  int G = touched < p.nunits ? touched : p.nunits;
+  int G = fixed_groups > 0 ? fixed_groups : touched;
+  G = G < p.nunits ? G : p.nunits;
   G = G < groups_cap ? G : groups_cap;
   G = G < kMaxGroups ? G : kMaxGroups;
   p.G = G < 1 ? 1 : G;
@@ -877,22 +886,13 @@ size_t site_scratch_bytes(int nrows, int roi, int nsheds) {
 
 int site_key_rows(int nrows, int roi) { return 1; }
 
-siting_status launch_site(const Geometry &g, const int32_t *obs,
-                          const uint64_t *sheds, const SiteBuffers &b,
-                          int device, hipStream_t s) {
+siting_status site_make_args(const Geometry &g, const int32_t *obs,
+                             const uint64_t *sheds, const SiteBuffers &b, int slot,
+                             SiteArgs *out, size_t *lds) {
   const int n = g.p.nrows, roi = g.p.roi, nsheds = g.d.nwanted;
-  int cus = 0;
-  SITING_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
-                                   device));
-  // The grid barrier needs every workgroup of the loop resident at once; a
-  // grid of at most half the CUs (one 1024-thread group per CU) is, since any
-  // other kernel holding CUs when it starts does not wait on it and retires.
-  // A plain launch is used: a cooperative launch reserves a device-wide
-  // resource that a second process on the same GPU cannot get.
-  const Plan p = make_plan(n, roi, nsheds, cus / 2 > 0 ? cus / 2 : 1);
+  const Plan p = make_plan(n, roi, nsheds, slot, slot);
   Scratch sc;
   scratch_layout(p, nsheds, b.scratch, &sc);
-
   SiteArgs a{};
   a.obs = obs;
   a.sheds = sheds;
@@ -936,25 +936,70 @@ siting_status launch_site(const Geometry &g, const int32_t *obs,
   a.o_stk = p.o_stk;
   a.o_off = p.o_off;
   a.off_lds = p.off_lds;
+  a.delta_lds = p.delta_lds;
   a.o_c = p.o_c;
   a.o_pos = p.o_pos;
   a.o_t = p.o_t;
   a.o_used = p.o_used;
   a.npts = (double)n * (double)n;
+  *out = a;
+  if (lds) *lds = p.lds;
+  return SITING_OK;
+}
 
-  const int tb = 256, nbk = (nsheds + tb - 1) / tb;
-  hipLaunchKernelGGL(site_zero_counts, dim3((p.nbuckets + 255) / 256), dim3(256), 0, s,
-                     sc.bcount, p.nbuckets);
+siting_status site_launch_prep(const SiteArgs &a, hipStream_t s) {
+  const int tb = 256, nbk = (a.nsheds + tb - 1) / tb;
+  const int nchunks = (a.nbuckets + 1023) / 1024;
+  hipLaunchKernelGGL(site_zero_counts, dim3((a.nbuckets + 255) / 256), dim3(256), 0, s,
+                     a.bcount, a.nbuckets);
   hipLaunchKernelGGL(site_bucket_count, dim3(nbk), dim3(tb), 0, s, a);
-  hipLaunchKernelGGL(site_scan_chunks, dim3(p.nchunks), dim3(1024), 0, s, a);
-  hipLaunchKernelGGL(site_scan_apply, dim3(p.nchunks), dim3(1024), 0, s, a);
+  hipLaunchKernelGGL(site_scan_chunks, dim3(nchunks), dim3(1024), 0, s, a);
+  hipLaunchKernelGGL(site_scan_apply, dim3(nchunks), dim3(1024), 0, s, a);
   hipLaunchKernelGGL(site_bucket_scatter, dim3(nbk), dim3(tb), 0, s, a);
-  hipLaunchKernelGGL(site_first_pass, dim3((nsheds + 3) / 4), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(site_first_pass, dim3((a.nsheds + 3) / 4), dim3(256), 0, s, a);
   SITING_HIP(hipGetLastError());
-  if (p.delta_lds)
-    hipLaunchKernelGGL(site_loop<true>, dim3(p.G), dim3(kLoopThreads), p.lds, s, a);
-  else
-    hipLaunchKernelGGL(site_loop<false>, dim3(p.G), dim3(kLoopThreads), p.lds, s, a);
+  return SITING_OK;
+}
+
+siting_status launch_site(const Geometry &g, const int32_t *obs,
+                          const uint64_t *sheds, const SiteBuffers &b,
+                          int device, hipStream_t s) {
+  int cus = 0;
+  SITING_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                   device));
+  // The grid barrier needs every workgroup of the loop resident at once; a
+  // grid of at most half the CUs (one 1024-thread group per CU) is, since any
+  // other kernel holding CUs when it starts does not wait on it and retires.
+  // A plain launch is used: a cooperative launch reserves a device-wide
+  // resource that a second process on the same GPU cannot get.
+  const int cap = cus / 2 > 0 ? cus / 2 : 1;
+  const Plan p = make_plan(g.p.nrows, g.p.roi, g.d.nwanted, cap);
+  SiteArgs a;
+  size_t lds = 0;
+  siting_status st = site_make_args(g, obs, sheds, b, p.G, &a, &lds);
+  if (st != SITING_OK) return st;
+  if ((st = site_launch_prep(a, s)) != SITING_OK) return st;
+  hipLaunchKernelGGL(site_loop, dim3(a.G), dim3(kLoopThreads), lds, s, a);
+  SITING_HIP(hipGetLastError());
+  return SITING_OK;
+}
+
+__global__ void site_zero_ctl(unsigned int *ctl, int words) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < words) ctl[i] = 0;
+}
+
+size_t site_queue_ctl_words(int nslots) { return 1 + 3 * (size_t)nslots; }
+
+siting_status site_launch_queue(const SiteArgs *jobs_dev, int njobs, int slot,
+                                int nslots, size_t lds, unsigned int *ctl,
+                                hipStream_t s) {
+  const int words = (int)site_queue_ctl_words(nslots);
+  hipLaunchKernelGGL(site_zero_ctl, dim3((words + 255) / 256), dim3(256), 0, s, ctl,
+                     words);
+  unsigned int *queue = ctl, *slot_job = ctl + 1, *slot_bar = ctl + 1 + 2 * nslots;
+  hipLaunchKernelGGL(site_loop_queue, dim3(nslots * slot), dim3(kLoopThreads), lds, s,
+                     jobs_dev, njobs, slot, queue, slot_job, slot_bar);
   SITING_HIP(hipGetLastError());
   return SITING_OK;
 }
