@@ -78,7 +78,8 @@ struct siting_job {
   double *rdiv = nullptr;
   hipEvent_t ev_start[4] = {};
   hipEvent_t ev_end[4] = {};
-  bool ran[4] = {};
+  bool ran[4] = {};          // stage events recorded
+  bool site_state = false;   // SITE state written (job or batch run)
 };
 
 namespace {
@@ -277,6 +278,7 @@ siting_status siting_job_upload_dem(siting_job *j, const uint16_t *elevs) {
     st_ = record(j->ev_end[idx], j->stream);                         \
     if (st_ != SITING_OK) return st_;                                \
     j->ran[idx] = true;                                              \
+    if (idx == 3) j->site_state = true;                              \
     return SITING_OK;                                                \
   } while (0)
 
@@ -350,7 +352,7 @@ siting_status siting_job_stats(siting_job *j, siting_stats *st) {
   float *ms[4] = {&st->ms_vix, &st->ms_findmax, &st->ms_viewshed, &st->ms_site};
   for (int i = 0; i < 4; i++)
     if (j->ran[i]) SITING_HIP(hipEventElapsedTime(ms[i], j->ev_start[i], j->ev_end[i]));
-  if (j->ran[3]) {
+  if (j->site_state) {
     SiteState s;
     SITING_HIP(hipMemcpy(&s, j->site.state, sizeof s, hipMemcpyDeviceToHost));
     st->nused = s.nused;
@@ -700,7 +702,7 @@ siting_status siting_batch_create(int device, int njobs, const siting_params *pa
                                       &args[(size_t)q], &lds);
     if (st != SITING_OK) return fail(st);
     b->lds = lds > b->lds ? lds : b->lds;
-    j->ran[3] = true;  // stats read the SITE state
+    j->site_state = true;  // stats read the SITE state
   }
   if ((e = hipMalloc(&b->args_dev, sizeof(SiteArgs) * (size_t)njobs)) != hipSuccess ||
       (e = hipMalloc(&b->ctl, 4 * site_queue_ctl_words(b->nslots))) != hipSuccess)
