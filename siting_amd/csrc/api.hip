@@ -664,7 +664,7 @@ siting_status siting_batch_create(int device, int njobs, const siting_params *pa
   int cus = 0;
   SITING_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
   const int nstreams = opts && opts->streams > 0 ? opts->streams : 4;
-  b->slot = opts && opts->slot > 0 ? opts->slot : 16;
+  b->slot = opts && opts->slot > 0 ? opts->slot : 32;
   if (b->slot > cus) b->slot = cus;
   b->nslots = opts && opts->nslots > 0 ? opts->nslots : cus / b->slot;
   if (b->nslots * b->slot > cus) b->nslots = cus / b->slot;

@@ -151,11 +151,12 @@ struct SiteArgs {
   int32_t *sT;       // [nsheds] area state of workgroups in global mode
   uint8_t *sused;    // [nsheds] used state of workgroups in global mode
   uint64_t *nb;      // [kSiteMaxBatch][nr][nwpr + 1] bits each pick newly covered
+  unsigned int *sseen;  // [G][nsheds/32 + 1] seen bitmaps of global-mode groups
   int n, roi, nr, nwpr, cw, nsheds, intervis;
   int cs, ncr, Q, nunits, G, upw, kpu, nbuckets, cap;
   // dynamic LDS layout (byte offsets)
   int o_delta, o_ukey, o_upos, o_dirty, o_off, o_c, o_pos, o_t, o_used, o_sk, o_sp,
-      o_stk;
+      o_stk, o_seen;
   int off_lds;    // unit cell offsets kept in LDS (else read from bstart)
   int delta_lds;  // the winner's shifted shed staged in LDS (else recomputed)
   double npts;

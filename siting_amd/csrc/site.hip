@@ -47,7 +47,7 @@ namespace {
 constexpr int kLoopThreads = 1024;
 constexpr int kLoopWaves = kLoopThreads / kWave;
 constexpr int kMaxList = 1024;
-constexpr size_t kLdsBudget = 140 * 1024;  // dynamic LDS of the loop kernel
+constexpr size_t kLdsBudget = 132 * 1024;  // dynamic LDS of the loop kernel
 constexpr int kMaxGroups = 128;
 constexpr int kMaxBatch = kSiteMaxBatch;
 
@@ -330,6 +330,32 @@ __device__ __forceinline__ int wave_lost_bits(const uint64_t *sh, int cx, int cy
   return wave_sum_i32(s);
 }
 
+// Maximum of a 64-bit value over the wavefront, in every lane, with DPP row
+// operations (no LDS traffic): quad swaps, half-row and row mirrors, then the
+// row broadcasts that fold rows 0-3 into lane 63.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ uint64_t dpp_max_step(uint64_t v) {
+  const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  const uint32_t olo = (uint32_t)__builtin_amdgcn_update_dpp((int)lo, (int)lo, kCtrl,
+                                                             kRowMask, 0xf, false);
+  const uint32_t ohi = (uint32_t)__builtin_amdgcn_update_dpp((int)hi, (int)hi, kCtrl,
+                                                             kRowMask, 0xf, false);
+  const uint64_t o = ((uint64_t)ohi << 32) | olo;
+  return o > v ? o : v;
+}
+
+__device__ __forceinline__ uint64_t wave_max_u64_dpp(uint64_t v) {
+  v = dpp_max_step<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v = dpp_max_step<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+  v = dpp_max_step<0x141, 0xf>(v);  // row_half_mirror
+  v = dpp_max_step<0x140, 0xf>(v);  // row_mirror
+  v = dpp_max_step<0x142, 0xa>(v);  // row_bcast:15 into rows 1, 3
+  v = dpp_max_step<0x143, 0xc>(v);  // row_bcast:31 into rows 2, 3
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // Best and second best (key, position) over the unused candidates of owned
 // unit ul; one wavefront.
 struct Top2 {
@@ -369,6 +395,10 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   __shared__ int list_i[kMaxList];
   __shared__ int list_acc[kMaxList];
   __shared__ int list_n;
+  __shared__ int rng_lo[kMaxList];
+  __shared__ int rng_end[kMaxList];
+  __shared__ int rng_n;
+  __shared__ int wsum[kLoopWaves];
   __shared__ uint64_t bw_key[kMaxBatch];
   __shared__ uint32_t bw_pos[kMaxBatch];
   __shared__ int bw_n;
@@ -400,6 +430,10 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   const uint32_t *P = in_lds ? (const uint32_t *)(smem + a.o_pos) : a.spos + B0;
   int32_t *T = in_lds ? (int32_t *)(smem + a.o_t) : a.sT + B0;
   uint8_t *U = in_lds ? (uint8_t *)(smem + a.o_used) : a.sused + B0;
+  // seen-this-round bits over the owned candidates (global mode: a slice of
+  // a global bitmap, one word per 32 candidates from this group's start)
+  unsigned int *seen = in_lds ? (unsigned int *)(smem + a.o_seen)
+                              : a.sseen + (size_t)g * ((a.nsheds + 31) / 32 + 1);
   for (int i = tid; i < cnt; i += kLoopThreads) {
     const int c = a.sid[B0 + i];
     if (in_lds) {
@@ -421,6 +455,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     }
   if (tid == 0) list_n = 0;
   for (int ul = tid; ul < upw; ul += kLoopThreads) dirty[ul] = 0;
+  for (int i = tid; i < (cnt + 31) / 32; i += kLoopThreads) seen[i] = 0;
   __syncthreads();
 
   auto unit_top2 = [&](int ul) {
@@ -495,18 +530,11 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       lane_best(lv, lu);
       int k = 0;
       while (k < kmax) {
-        uint64_t bv = lv;
-        int bu = lu;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-          const uint64_t ov = __shfl_xor(bv, o, kWave);
-          const int ou = __shfl_xor(bu, o, kWave);
-          if (ov > bv) {
-            bv = ov;
-            bu = ou;
-          }
-        }
+        const uint64_t bv = wave_max_u64_dpp(lv);
         if ((bv >> 32) == 0) break;       // no candidate adds area
+        // the lane that holds it (keys are distinct)
+        const uint64_t hit = __ballot(lv == bv);
+        const int bu = __shfl(lu, (int)__builtin_ctzll(hit), kWave);
         const int t = stk[bu];
         if (t >= 2) break;                // the unit's next is unknown
         const uint32_t pos = sp[2 * bu + t];
@@ -639,11 +667,64 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       grid_sync(&a.st->barrier, bar_target, (unsigned)G);
     }
 
-    // ---- update lists of picks 0..kc-1 among this workgroup's candidates
-    for (int base = 0; base < cnt; base += kMaxList) {
-      if (tid < min(kMaxList, cnt - base)) {  // cpp:540-544, once per list
-        const int i = base + tid;
-        if (!U[i]) {
+    // ---- update lists of picks 0..kc-1 among this workgroup's candidates.
+    // Ranges: for each pick, the cells of owned units inside the pick's
+    // 2*roi window (unit/cell offsets); their concatenation is walked in
+    // windows of kMaxList items; a candidate in several ranges is listed
+    // once (seen bitmap, cleared at the end of the round).
+    if (tid == 0) rng_n = 0;
+    __syncthreads();
+    for (int t = tid; t < kc * upw; t += kLoopThreads) {
+      const int p = t / upw, ul = t - p * upw;
+      const int u = ul * G + g;
+      if (u >= nunits) continue;
+      const int r = u / Q, q = u - r * Q;
+      const int wx = (int)(bw_pos[p] >> 16), wy = (int)(bw_pos[p] & 0xffffu);
+      if (r < max(0, (wx - 2 * roi) / a.cs) || r > (wx + 2 * roi) / a.cs) continue;
+      const int cclo = max(0, (wy - 2 * roi) / a.cs);
+      const int cchi = min(a.ncr - 1, (wy + 2 * roi) / a.cs);
+      const int klo = cclo <= q ? 0 : (cclo - q + Q - 1) / Q;
+      const int khi = cchi < q ? -1 : (cchi - q) / Q;
+      if (klo > khi) continue;
+      const int lo = OFF(ul, klo), hi = OFF(ul, khi + 1);
+      if (lo >= hi) continue;
+      const int at = atomicAdd(&rng_n, 1);
+      if (at < kMaxList) {
+        rng_lo[at] = lo;
+        rng_end[at] = hi - lo;
+      }
+    }
+    __syncthreads();
+    int nrng = rng_n;
+    if (nrng > kMaxList) {  // too many ranges: scan every owned candidate
+      __syncthreads();
+      if (tid == 0) {
+        rng_lo[0] = 0;
+        rng_end[0] = cnt;
+        rng_n = 1;
+      }
+      __syncthreads();
+      nrng = 1;
+    }
+    {
+      const int v = tid < nrng ? rng_end[tid] : 0;
+      const int inc = block_inclusive_scan(v, wsum);
+      __syncthreads();
+      if (tid < nrng) rng_end[tid] = inc;
+    }
+    __syncthreads();
+    const int total = nrng > 0 ? rng_end[nrng - 1] : 0;
+    for (int w0 = 0; w0 < total; w0 += kMaxList) {
+      const int item = w0 + tid;
+      if (item < total) {  // cpp:540-544, once per list
+        int lo = 0, hi = nrng - 1;  // first range whose end exceeds item
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (rng_end[mid] > item) hi = mid; else lo = mid + 1;
+        }
+        const int i = rng_lo[lo] + item - (lo > 0 ? rng_end[lo - 1] : 0);
+        const unsigned int bit = 1u << (i & 31);
+        if (!U[i] && !(atomicOr(&seen[i >> 5], bit) & bit)) {
           const uint32_t ps = P[i];
           const int cx = (int)(ps >> 16), cy = (int)(ps & 0xffffu);
           int m = 0;
@@ -699,6 +780,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       if (tid == 0) list_n = 0;
       __syncthreads();
     }
+    for (int i = tid; i < (cnt + 31) / 32; i += kLoopThreads) seen[i] = 0;
     if (stop) break;
     __syncthreads();
     for (int ul = wid; ul < upw; ul += kLoopWaves)
@@ -763,7 +845,7 @@ struct Plan {
   int cs, ncr, Q, nunits, G, upw, kpu, nbuckets, nchunks, cap, delta_lds, off_lds;
   size_t lds, nbwords;
   int o_delta, o_ukey, o_upos, o_dirty, o_off, o_c, o_pos, o_t, o_used, o_sk, o_sp,
-      o_stk;
+      o_stk, o_seen;
 };
 
 int env_q() {
@@ -826,7 +908,7 @@ Plan make_plan(int n, int roi, int nsheds, int groups_cap, int fixed_groups = 0)
   o = align_up(o, 16);
   const size_t fixed = o;
   const size_t room = kLdsBudget > fixed ? kLdsBudget - fixed : 0;
-  int cap = (int)(room / 13);
+  int cap = (int)(room / 14);  // id, pos, area, used, seen bit (rounded up)
   cap = cap > 16 ? (cap - 16) / 16 * 16 : 0;
   p.cap = cap;
   p.o_c = (int)o;
@@ -837,6 +919,9 @@ Plan make_plan(int n, int roi, int nsheds, int groups_cap, int fixed_groups = 0)
   o += 4 * (size_t)cap;
   p.o_used = (int)o;
   o += (size_t)cap;
+  o = align_up(o, 16);
+  p.o_seen = (int)o;
+  o += 4 * (((size_t)cap + 31) / 32);
   p.lds = align_up(o, 16);
   return p;
 }
@@ -848,6 +933,7 @@ struct Scratch {
   uint64_t *keys;
   uint32_t *kpos;
   uint64_t *nb;
+  unsigned int *sseen;
 };
 
 size_t scratch_layout(const Plan &p, int nsheds, uint8_t *base, Scratch *out) {
@@ -869,6 +955,8 @@ size_t scratch_layout(const Plan &p, int nsheds, uint8_t *base, Scratch *out) {
   s.sT = (int32_t *)take(4 * (size_t)nsheds);
   s.sused = (uint8_t *)take((size_t)nsheds);
   s.nb = (uint64_t *)take(8 * p.nbwords);
+  // global-mode seen bitmaps: one slice of ceil(nsheds/32)+1 words per group
+  s.sseen = (unsigned int *)take(4 * ((size_t)nsheds / 32 + 2) * kMaxGroups);
   s.keys = (uint64_t *)take(8 * 4 * (size_t)p.nunits);
   s.kpos = (uint32_t *)take(4 * 4 * (size_t)p.nunits);
   if (out) *out = s;
@@ -911,6 +999,7 @@ siting_status site_make_args(const Geometry &g, const int32_t *obs,
   a.sT = sc.sT;
   a.sused = sc.sused;
   a.nb = sc.nb;
+  a.sseen = sc.sseen;
   a.n = n;
   a.roi = roi;
   a.nr = g.d.nr;
@@ -941,6 +1030,7 @@ siting_status site_make_args(const Geometry &g, const int32_t *obs,
   a.o_pos = p.o_pos;
   a.o_t = p.o_t;
   a.o_used = p.o_used;
+  a.o_seen = p.o_seen;
   a.npts = (double)n * (double)n;
   *out = a;
   if (lds) *lds = p.lds;
