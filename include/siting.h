@@ -205,6 +205,65 @@ siting_status siting_batch_elapsed_ms(siting_batch *b, float *ms);
 /* Borrowed handle of job i, for downloads and stats (valid until destroy). */
 siting_status siting_batch_job(siting_batch *b, int job, siting_job **out);
 
+/* ---- serving -----------------------------------------------------------
+ * A request loop on one GPU: submitted jobs are admitted as device memory
+ * and SITE workgroups free up, run alongside the jobs already on the device
+ * (each on its own stream, all data resident in HBM) and retire into host
+ * result buffers.  First-come first-served with backfill of smaller requests
+ * while the head of the queue waits (up to backfill_ms).  A job's results
+ * are bit-identical to running it alone. */
+
+typedef struct siting_server siting_server;
+
+typedef struct {
+  size_t hbm_bytes;   /* device memory for jobs (0: 80% of free memory) */
+  int max_active;     /* jobs on the device at once (0: 8) */
+  int site_groups;    /* workgroups of each job's SITE loop (0: 32) */
+  int backfill_ms;    /* how long smaller requests may pass a blocked head (0: 200) */
+} siting_server_opts;
+
+typedef enum {
+  SITING_JOB_QUEUED = 0,
+  SITING_JOB_RUNNING = 1,
+  SITING_JOB_DONE = 2,
+  SITING_JOB_FAILED = 3
+} siting_job_state;
+
+typedef struct {
+  siting_job_state state;
+  siting_status status;   /* why it failed, when it did */
+  siting_stats stats;     /* SITE counters (stage times are not recorded) */
+  double t_submit, t_admit, t_done; /* seconds on the server's clock */
+} siting_job_info;
+
+typedef struct {
+  int queued, running, groups_in_use;
+  size_t hbm_bytes;
+} siting_server_load_info;
+
+siting_status siting_server_create(int device, const siting_server_opts *opts,
+                                   siting_server **out);
+/* Drops queued requests (FAILED, SITING_EBUSY), waits for running ones. */
+void siting_server_destroy(siting_server *s);
+/* Copies the DEM.  Rejects malformed parameters (SITING_EINVAL /
+ * SITING_ESMALLBLOCK) and jobs larger than the whole budget
+ * (SITING_ENOMEM) at once; the server keeps running. */
+siting_status siting_server_submit(siting_server *s, const siting_params *p,
+                                   const uint16_t *elevs, uint64_t *id);
+siting_status siting_server_info(siting_server *s, uint64_t id, siting_job_info *info);
+/* Blocks until done or failed (timeout_s < 0: no limit; SITING_EBUSY on
+ * timeout). */
+siting_status siting_server_wait(siting_server *s, uint64_t id, double timeout_s,
+                                 siting_job_info *info);
+/* selected: uint8[nwanted]; obs: int32[nwanted] (either may be NULL). */
+siting_status siting_server_result(siting_server *s, uint64_t id, uint8_t *selected,
+                                   size_t nsel, int32_t *obs, size_t nobs);
+/* Removes a queued request (SITING_EBUSY if it already started). */
+siting_status siting_server_cancel(siting_server *s, uint64_t id);
+/* Frees a finished request's results. */
+siting_status siting_server_release(siting_server *s, uint64_t id);
+siting_status siting_server_load(siting_server *s, siting_server_load_info *out);
+
 /* ---- terrain input (raw uint16 format, siting.cpp:726-730) ------------ */
 
 /* Exactly nrows*nrows native-endian uint16 values; SITING_EIO if short. */

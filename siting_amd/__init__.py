@@ -118,6 +118,16 @@ _SIGS = {
     "siting_batch_sync": (_I, [_VP]),
     "siting_batch_elapsed_ms": (_I, [_VP, ctypes.POINTER(ctypes.c_float)]),
     "siting_batch_job": (_I, [_VP, _I, ctypes.POINTER(_VP)]),
+    "siting_server_create": (_I, [_I, _VP, ctypes.POINTER(_VP)]),
+    "siting_server_destroy": (None, [_VP]),
+    "siting_server_submit": (_I, [_VP, _VP, _VP, ctypes.POINTER(ctypes.c_uint64)]),
+    "siting_server_info": (_I, [_VP, ctypes.c_uint64, _VP]),
+    "siting_server_wait": (_I, [_VP, ctypes.c_uint64, ctypes.c_double, _VP]),
+    "siting_server_result": (_I, [_VP, ctypes.c_uint64, _VP, ctypes.c_size_t, _VP,
+                                  ctypes.c_size_t]),
+    "siting_server_cancel": (_I, [_VP, ctypes.c_uint64]),
+    "siting_server_release": (_I, [_VP, ctypes.c_uint64]),
+    "siting_server_load": (_I, [_VP, _VP]),
 }
 
 
@@ -385,6 +395,95 @@ class Batch:
         h = ctypes.c_void_p()
         _check(lib().siting_batch_job(self._h, i, ctypes.byref(h)))
         return _JobView(h, self.params[i])
+
+
+# ---- serving ---------------------------------------------------------------
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = range(4)
+
+
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("hbm_bytes", ctypes.c_size_t), ("max_active", ctypes.c_int),
+                ("site_groups", ctypes.c_int), ("backfill_ms", ctypes.c_int)]
+
+
+class JobInfo(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int), ("status", ctypes.c_int), ("stats", Stats),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_done", ctypes.c_double)]
+
+
+class LoadInfo(ctypes.Structure):
+    _fields_ = [("queued", ctypes.c_int), ("running", ctypes.c_int),
+                ("groups_in_use", ctypes.c_int), ("hbm_bytes", ctypes.c_size_t)]
+
+
+class Server:
+    """Request loop on one GPU (see siting.h, serving)."""
+
+    def __init__(self, device=0, hbm_bytes=0, max_active=0, site_groups=0, backfill_ms=0):
+        opts = ServerOpts(hbm_bytes, max_active, site_groups, backfill_ms)
+        h = ctypes.c_void_p()
+        _check(lib().siting_server_create(device, ctypes.byref(opts), ctypes.byref(h)))
+        self._h = h
+        self._params = {}
+
+    def close(self):
+        if self._h:
+            lib().siting_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, params: Params, elevs) -> int:
+        e = np.ascontiguousarray(elevs, dtype=np.uint16).reshape(-1)
+        if e.size != params.nrows ** 2:
+            raise SitingError(EINVAL, "DEM size mismatch")
+        jid = ctypes.c_uint64()
+        _check(lib().siting_server_submit(self._h, ctypes.byref(params), _ptr(e),
+                                          ctypes.byref(jid)))
+        self._params[jid.value] = params
+        return jid.value
+
+    def info(self, jid) -> JobInfo:
+        info = JobInfo()
+        _check(lib().siting_server_info(self._h, jid, ctypes.byref(info)))
+        return info
+
+    def wait(self, jid, timeout_s=-1.0) -> JobInfo:
+        info = JobInfo()
+        _check(lib().siting_server_wait(self._h, jid, timeout_s, ctypes.byref(info)))
+        return info
+
+    def result(self, jid):
+        d = derive(self._params[jid])
+        sel = np.empty(d.nwanted, np.uint8)
+        obs = np.empty(d.nwanted, np.int32)
+        _check(lib().siting_server_result(self._h, jid, _ptr(sel), sel.size, _ptr(obs),
+                                          obs.size))
+        return sel, obs
+
+    def cancel(self, jid):
+        _check(lib().siting_server_cancel(self._h, jid))
+
+    def release(self, jid):
+        _check(lib().siting_server_release(self._h, jid))
+        self._params.pop(jid, None)
+
+    def load(self) -> LoadInfo:
+        li = LoadInfo()
+        _check(lib().siting_server_load(self._h, ctypes.byref(li)))
+        return li
 
 
 # ---- terrain ---------------------------------------------------------------

@@ -7,7 +7,7 @@
 #include <mutex>
 #include <vector>
 
-#include "common.hpp"
+#include "job.hpp"
 
 namespace siting {
 
@@ -62,25 +62,7 @@ using namespace siting;
 // ---------------------------------------------------------------------------
 // job
 // ---------------------------------------------------------------------------
-struct siting_job {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  Geometry g{};
-  void *base = nullptr;
-  size_t bytes = 0;
-  uint16_t *elevs = nullptr;
-  uint8_t *vix = nullptr;
-  int32_t *obs = nullptr;
-  uint64_t *sheds = nullptr;
-  SiteBuffers site{};
-  float *recip = nullptr;
-  uint32_t *jump = nullptr;
-  double *rdiv = nullptr;
-  hipEvent_t ev_start[4] = {};
-  hipEvent_t ev_end[4] = {};
-  bool ran[4] = {};          // stage events recorded
-  bool site_state = false;   // SITE state written (job or batch run)
-};
+
 
 namespace {
 
@@ -172,15 +154,16 @@ int siting_device_count(void) {
   return c;
 }
 
-siting_status siting_job_bytes(const siting_params *p, size_t *bytes) {
-  siting_derived d;
-  siting_status st = derive(p, &d);
-  if (st != SITING_OK) return st;
-  if (bytes) *bytes = layout_for(*p, d).total;
-  return SITING_OK;
+}  // extern "C"
+
+namespace siting {
+
+size_t job_bytes(const siting_params &p, const siting_derived &d) {
+  return layout_for(p, d).total;
 }
 
-siting_status siting_job_create(int device, const siting_params *p, siting_job **out) {
+siting_status job_create_in(int device, const siting_params *p, void *mem,
+                            hipStream_t stream, siting_job **out) {
   if (!out) return SITING_EINVAL;
   *out = nullptr;
   siting_derived d;
@@ -202,13 +185,24 @@ siting_status siting_job_create(int device, const siting_params *p, siting_job *
     siting_job_destroy(j);
     return s;
   };
-  hipError_t e = hipMalloc(&j->base, L.total);
-  if (e != hipSuccess) {
-    j->base = nullptr;
-    return fail(hip_fail(e, "hipMalloc(job)", __FILE__, __LINE__));
+  hipError_t e = hipSuccess;
+  if (mem) {
+    j->base = mem;
+    j->owns_mem = false;
+  } else {
+    e = hipMalloc(&j->base, L.total);
+    if (e != hipSuccess) {
+      j->base = nullptr;
+      return fail(hip_fail(e, "hipMalloc(job)", __FILE__, __LINE__));
+    }
   }
-  e = hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) return fail(hip_fail(e, "hipStreamCreate", __FILE__, __LINE__));
+  if (stream) {
+    j->stream = stream;
+    j->owns_stream = false;
+  } else {
+    e = hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(hip_fail(e, "hipStreamCreate", __FILE__, __LINE__));
+  }
   for (int i = 0; i < 4; i++) {
     if ((e = hipEventCreate(&j->ev_start[i])) != hipSuccess ||
         (e = hipEventCreate(&j->ev_end[i])) != hipSuccess)
@@ -245,6 +239,22 @@ siting_status siting_job_create(int device, const siting_params *p, siting_job *
   return SITING_OK;
 }
 
+}  // namespace siting
+
+extern "C" {
+
+siting_status siting_job_bytes(const siting_params *p, size_t *bytes) {
+  siting_derived d;
+  siting_status st = derive(p, &d);
+  if (st != SITING_OK) return st;
+  if (bytes) *bytes = layout_for(*p, d).total;
+  return SITING_OK;
+}
+
+siting_status siting_job_create(int device, const siting_params *p, siting_job **out) {
+  return job_create_in(device, p, nullptr, nullptr, out);
+}
+
 void siting_job_destroy(siting_job *j) {
   if (!j) return;
   (void)hipSetDevice(j->device);
@@ -253,8 +263,8 @@ void siting_job_destroy(siting_job *j) {
     if (j->ev_start[i]) (void)hipEventDestroy(j->ev_start[i]);
     if (j->ev_end[i]) (void)hipEventDestroy(j->ev_end[i]);
   }
-  if (j->stream) (void)hipStreamDestroy(j->stream);
-  if (j->base) (void)hipFree(j->base);
+  if (j->stream && j->owns_stream) (void)hipStreamDestroy(j->stream);
+  if (j->base && j->owns_mem) (void)hipFree(j->base);
   delete j;
 }
 
@@ -292,7 +302,8 @@ siting_status siting_job_viewshed(siting_job *j) {
   JOB_STAGE(2, launch_viewshed(j->g, j->elevs, j->obs, j->sheds, j->rdiv, j->stream));
 }
 siting_status siting_job_site(siting_job *j) {
-  JOB_STAGE(3, launch_site(j->g, j->obs, j->sheds, j->site, j->device, j->stream));
+  JOB_STAGE(3, launch_site(j->g, j->obs, j->sheds, j->site, j->device, j->stream,
+                           j->site_groups));
 }
 
 siting_status siting_job_run(siting_job *j) {

@@ -167,9 +167,12 @@ struct SiteArgs {
 size_t site_scratch_bytes(int nrows, int roi, int nsheds);
 int site_key_rows(int nrows, int roi);
 
+// groups_cap: most workgroups the loop may use (0: half the CUs)
 siting_status launch_site(const Geometry &g, const int32_t *obs,
                           const uint64_t *sheds, const SiteBuffers &b,
-                          int device, hipStream_t s);
+                          int device, hipStream_t s, int groups_cap = 0);
+// Workgroups launch_site's loop will use for this job under groups_cap.
+int site_loop_groups(const Geometry &g, int device_cus, int groups_cap);
 
 // Batched SITE: site_make_args plans one job for a work-queue slot of `slot`
 // workgroups (no launch); site_launch_prep enqueues its preparation kernels;

@@ -1051,9 +1051,15 @@ siting_status site_launch_prep(const SiteArgs &a, hipStream_t s) {
   return SITING_OK;
 }
 
+int site_loop_groups(const Geometry &g, int device_cus, int groups_cap) {
+  int cap = device_cus / 2 > 0 ? device_cus / 2 : 1;
+  if (groups_cap > 0 && groups_cap < cap) cap = groups_cap;
+  return make_plan(g.p.nrows, g.p.roi, g.d.nwanted, cap).G;
+}
+
 siting_status launch_site(const Geometry &g, const int32_t *obs,
                           const uint64_t *sheds, const SiteBuffers &b,
-                          int device, hipStream_t s) {
+                          int device, hipStream_t s, int groups_cap) {
   int cus = 0;
   SITING_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
                                    device));
@@ -1062,7 +1068,8 @@ siting_status launch_site(const Geometry &g, const int32_t *obs,
   // other kernel holding CUs when it starts does not wait on it and retires.
   // A plain launch is used: a cooperative launch reserves a device-wide
   // resource that a second process on the same GPU cannot get.
-  const int cap = cus / 2 > 0 ? cus / 2 : 1;
+  int cap = cus / 2 > 0 ? cus / 2 : 1;
+  if (groups_cap > 0 && groups_cap < cap) cap = groups_cap;
   const Plan p = make_plan(g.p.nrows, g.p.roi, g.d.nwanted, cap);
   SiteArgs a;
   size_t lds = 0;
