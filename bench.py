@@ -17,6 +17,12 @@ peak 5000 and sigma 80 cells), DEMs in HBM before any timed region:
                 stage; bit-exactness against the CPU oracle; the oracle's
                 OpenMP variant timed on this host's cores (cpu_baseline).
 
+  served stream  Poisson arrivals at a fixed rate of jobs with edge
+                log-uniform in [256, 8192], roi in {25,50,100,200}, 64 targets,
+                bw 16, nwanted n^2/512, intervis 1 with p = 0.2, through the
+                request loop (admission by HBM need, jobs sharing the device):
+                throughput and p50/p99 latency (submit to results).
+
 Prints one JSON line; `value` is the batched throughput (terrain points per
 second over the whole batch).
 """
@@ -50,6 +56,57 @@ def uneven_batch(njobs=64, seed=2024):
                          ntests=int(rng.choice([32, 64])), bw=int(rng.choice([8, 16])),
                          nwanted0=n * n // 256, height=100, intervis=0, seed=5000 + j))
     return jobs
+
+
+def serve_stream(nreq=32, qps=8.0, seed=77):
+    """The served stream: Poisson arrivals at a fixed rate; one seeded draw per
+    request (edge log-uniform in [256, 8192], roi in {25,50,100,200}, 64
+    targets, bw 16, nwanted n^2/512, intervis 1 with p = 0.2)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    reqs, t = [], 0.0
+    for j in range(nreq):
+        t += float(rng.exponential(1.0 / qps))
+        n = int(round(math.exp(rng.uniform(math.log(256), math.log(8192)))))
+        reqs.append(dict(at=t, nrows=n, rough_h=0.8, hmax=20000,
+                         roi=int(rng.choice([25, 50, 100, 200])), ntests=64, bw=16,
+                         nwanted0=n * n // 512, height=100,
+                         intervis=int(rng.random() < 0.2), seed=9000 + j))
+    return reqs
+
+
+def run_serve(sa, np, nreq, qps):
+    import threading
+    reqs = serve_stream(nreq, qps)
+    dems = [make_dem(sa, w) for w in reqs]
+    log(f"serve: {len(reqs)} requests at {qps}/s, DEMs generated")
+    srv = sa.Server(max_active=8, site_groups=32)
+    ids = [None] * len(reqs)
+
+    def submitter():
+        t0 = time.perf_counter()
+        for i, w in enumerate(reqs):
+            delay = w["at"] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            ids[i] = srv.submit(params_of(sa, w), dems[i])
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    infos = [srv.wait(i, 600.0) for i in ids]
+    srv.close()
+    done = [x for x in infos if x.state == 2]
+    lat = np.array([x.t_done - x.t_submit for x in done])
+    qwait = np.array([x.t_admit - x.t_submit for x in done])
+    span = max(x.t_done for x in done) - min(x.t_submit for x in done)
+    pts = sum(w["nrows"] ** 2 for w, x in zip(reqs, infos) if x.state == 2)
+    return {"requests": len(reqs), "done": len(done), "qps_offered": qps,
+            "points_per_s": round(pts / span, 1), "jobs_per_s": round(len(done) / span, 3),
+            "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+            "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+            "p50_queue_s": round(float(np.percentile(qwait, 50)), 4),
+            "max_latency_s": round(float(lat.max()), 4)}
 
 
 def make_dem(sa, w):
@@ -88,6 +145,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-qps", type=float, default=8.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the "
@@ -190,6 +249,11 @@ def main():
                 "traffic": None}
     log(f"single job: {1000 * t_single:.2f} ms/step, stages {stage_ms}")
 
+    serve = None
+    if args.serve_requests > 0:
+        serve = run_serve(sa, np, args.serve_requests, args.serve_qps)
+        log(f"serve: {serve}")
+
     value = pts / t_batch
     result = {
         "metric": METRIC,
@@ -221,6 +285,8 @@ def main():
                        "site_hbm_gb_s": round(site_gbs, 2),
                        "site": {"nused": st.nused, "niter": st.niter, "sum_u": st.sum_u,
                                 "coverage": st.coverage}},
+        "serve": serve,
+        "p99_job_latency_s": serve["p99_latency_s"] if serve else None,
         "roofline": roofline,
         "cpu_baseline": None,
     }
