@@ -20,22 +20,41 @@
 //   every workgroup's candidates contiguously, unit by unit, cell by cell.
 //   site_first_pass  area = popcount(shed) for every candidate (cum is empty
 //                    and no mask applies), streaming the whole shed array once
-//   site_loop        persistent kernel, one grid barrier per iteration.  A
-//                    workgroup keeps its candidates' ids, positions, areas and
-//                    used flags in LDS (or, when they do not fit, in a global
-//                    slice of the same layout) and the best key of each of its
-//                    units.  Iteration: every workgroup reduces the unit keys
-//                    to the winner w (identically); the owner marks w used;
-//                    every workgroup stages w's shed, shifted into terrain
-//                    words, in LDS ("delta"), ORs its share of delta rows into
-//                    cum, recomputes the areas of the update candidates of its
-//                    own units against cum | delta (reading cum while others
-//                    OR the same delta in gives the same bits), and publishes
-//                    the new unit keys into the other half of a double-buffered
-//                    key array.
+//   site_loop        persistent kernel over G workgroups.  A workgroup keeps
+//                    its candidates' ids, positions, areas, used flags and a
+//                    seen-this-round bitmap in LDS (or, when they do not fit,
+//                    in a global slice of the same layout) and the two best
+//                    keys of each of its units.  A round:
+//                    1. every workgroup stages all published unit keys and
+//                       picks, identically, the longest prefix of the global
+//                       key order (at most 16 picks; 1 under intervis) whose
+//                       members are pairwise more than 2*roi apart.  None of
+//                       them is in another's update list, so the sequential
+//                       loop would pick exactly these, in this order, in as
+//                       many iterations (areas never grow when intervis is 0).
+//                    2. counters advance pick by pick; the 95% rule may end
+//                       the round after any pick.
+//                    3. the picks' sheds, shifted to terrain words, are ORed
+//                       into cum with atomicOr; the bits each OR newly set are
+//                       kept (nb), then one grid barrier.
+//                    4. every workgroup walks the candidates of its units that
+//                       lie within 2*roi of a pick (cell ranges), each once,
+//                       and lowers its area by the pick bits it loses
+//                       (intervis 0), or recomputes it against cum | delta
+//                       (intervis 1, one pick, no barrier in step 3).
+//                    5. dirty units re-rank their top two, publish them into
+//                       the other half of a double-buffered key array, grid
+//                       barrier.
+//   site_loop_queue  the same body for many jobs: slots of workgroups pop
+//                    job indices from a queue and run one job each.
 // Keys: (area << 32) | ~index, 0 for a used candidate, so the maximum key is
 // the largest area with the lowest index; a maximum with area 0 ends the
 // loop.  Each key travels with its candidate's position.
+//
+// Barriers are plain atomics on a counter, so every workgroup of a loop must
+// be resident at once: the host caps G at half the CUs (one 1024-thread
+// workgroup per CU fits the loop's LDS), and the work-queue launch sizes its
+// slots the same way.
 #include <cstdlib>
 
 #include "common.hpp"
