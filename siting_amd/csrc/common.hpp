@@ -159,6 +159,9 @@ struct SiteArgs {
       o_stk, o_seen;
   int off_lds;    // unit cell offsets kept in LDS (else read from bstart)
   int delta_lds;  // the winner's shifted shed staged in LDS (else recomputed)
+  int tau8;       // intervis 0: update candidates whose area is below tau8/8
+                  // of the round's largest become stale bounds instead of
+                  // being decremented (0: never)
   double npts;
 };
 

@@ -238,16 +238,17 @@ struct Winner {
   }
 };
 
-// New bits of candidate (cx, cy)'s shed against cum | winner, summed over
-// the wavefront (siting.cpp:456-491).  Work items are (shed row, terrain
-// word) pairs inside the terrain; lane l takes items l, l+64, ..., so a load
-// instruction reads consecutive words of the shed (rows are contiguous).
-// kBatch items per lane are loaded before any is used.
+// New bits of candidate (cx, cy)'s shed against cum | winner
+// (siting.cpp:456-491): this lane's share.  Work items are (shed row,
+// terrain word) pairs inside the terrain; lane gl of a group of gs lanes
+// takes items gl, gl+gs, ..., so a load instruction reads consecutive words
+// of the shed (rows are contiguous).  kBatch items per lane are loaded
+// before any is used.
 constexpr int kBatch = 4;
 
-__device__ __forceinline__ int wave_new_bits(const uint64_t *sh, int cx, int cy,
+__device__ __forceinline__ int lane_new_bits(const uint64_t *sh, int cx, int cy,
                                              const SiteArgs &a, const Winner &W,
-                                             int lane, bool use_w) {
+                                             int gl, int gs, bool use_w) {
   const int roi = a.roi, nwpr = a.nwpr, cw = a.cw, n = a.n;
   int w0c, fbc;
   col_split(cy - roi, w0c, fbc);
@@ -262,11 +263,11 @@ __device__ __forceinline__ int wave_new_bits(const uint64_t *sh, int cx, int cy,
   const bool fdiv = items < (1 << 21);
   const int xo = cx - roi, dro = xo - W.wx + roi;
   int s = 0;
-  for (int base = 0; base < items; base += kWave * kBatch) {
+  for (int base = 0; base < items; base += gs * kBatch) {
     uint64_t cur[kBatch], prv[kBatch], cv[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; u++) {
-      const int it = base + u * kWave + lane;
+      const int it = base + u * gs + gl;
       const bool ok = it < items;
       const int q = fdiv ? (int)(((float)it + 0.5f) * inv_nk) : it / nk;
       const int row = rlo + q, k = klo + (it - q * nk);
@@ -286,7 +287,13 @@ __device__ __forceinline__ int wave_new_bits(const uint64_t *sh, int cx, int cy,
       s += __popcll(moved & ~cv[u]);
     }
   }
-  return wave_sum_i32(s);
+  return s;
+}
+
+__device__ __forceinline__ int wave_new_bits(const uint64_t *sh, int cx, int cy,
+                                             const SiteArgs &a, const Winner &W,
+                                             int lane, bool use_w) {
+  return wave_sum_i32(lane_new_bits(sh, cx, cy, a, W, lane, kWave, use_w));
 }
 
 // Bits of candidate (cx, cy)'s shed that this round's picks newly covered,
@@ -421,6 +428,8 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   __shared__ uint64_t bw_key[kMaxBatch];
   __shared__ uint32_t bw_pos[kMaxBatch];
   __shared__ int bw_n;
+  __shared__ int bw_live;
+  __shared__ int bw_head;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wid = tid >> 6;
@@ -430,6 +439,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   const size_t S = (size_t)nr * nwpr;
   const int dw = nwpr + 1;
   const int kmax = a.intervis ? 1 : kMaxBatch;
+  const bool hybrid = !a.intervis && a.tau8 > 0;
 
   uint64_t *delta = (uint64_t *)(smem + a.o_delta);
   uint64_t *ukey = (uint64_t *)(smem + a.o_ukey);   // [upw][2]
@@ -477,11 +487,16 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   for (int i = tid; i < (cnt + 31) / 32; i += kLoopThreads) seen[i] = 0;
   __syncthreads();
 
+  // U[i]: bit 0 used, bit 1 stale (T[i] is then an upper bound of the area,
+  // not the area).  A published position carries the stale bit in bit 31
+  // (positions are below 2^31).
   auto unit_top2 = [&](int ul) {
     const int lo = OFF(ul, 0), hi = OFF(ul, kpu);
     Top2 t{0, 0, 0, 0};
-    for (int i = lo + lane; i < hi; i += kWave)
-      if (!U[i]) top2_push(t, cand_key(T[i], C[i]), P[i]);
+    for (int i = lo + lane; i < hi; i += kWave) {
+      const uint8_t u = U[i];
+      if (!(u & 1)) top2_push(t, cand_key(T[i], C[i]), P[i] | ((u & 2) ? 0x80000000u : 0u));
+    }
     t = wave_top2(t);
     if (lane == 0) {
       ukey[2 * ul] = t.k1;
@@ -548,15 +563,18 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       int lu;
       lane_best(lv, lu);
       int k = 0;
+      int head = 0;  // area (or bound) of the largest key
       while (k < kmax) {
         const uint64_t bv = wave_max_u64_dpp(lv);
         if ((bv >> 32) == 0) break;       // no candidate adds area
+        if (k == 0) head = (int)(bv >> 32);
         // the lane that holds it (keys are distinct)
         const uint64_t hit = __ballot(lv == bv);
         const int bu = __shfl(lu, (int)__builtin_ctzll(hit), kWave);
         const int t = stk[bu];
         if (t >= 2) break;                // the unit's next is unknown
         const uint32_t pos = sp[2 * bu + t];
+        if (pos >> 31) break;             // stale: its area is only a bound
         const int px = (int)(pos >> 16), py = (int)(pos & 0xffffu);
         bool near = false;
         if (lane < k) {
@@ -575,11 +593,19 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         }
         k++;
       }
-      if (lane == 0) bw_n = k;
+      if (lane == 0) {
+        bw_n = k;
+        bw_head = head;
+      }
     }
     __syncthreads();
     int k = bw_n;
-    if (k == 0) break;  // cpp:589-592
+    // no pick while the largest key is stale: the round only re-evaluates
+    const bool eval_only = k == 0 && bw_head > 0;
+    if (k == 0 && !eval_only) break;  // cpp:589-592
+    // update candidates below tau become stale; stale ones reaching tau are
+    // re-evaluated (areas only shrink, so a bound never falls below the area)
+    const int tau = hybrid ? (int)(((long long)bw_head * a.tau8) >> 3) : 0;
     if (nused + k > a.nsheds) {  // cannot happen: every pick retires a candidate
       niter = -1;
       break;
@@ -655,7 +681,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         const uint64_t v = W.word(dr, dk);
         if (v) a.cum[(size_t)x * cw + j] |= v;
       }
-    } else {
+    } else if (k > 0) {
       // every pick: OR its shifted shed into cum and keep the bits that were
       // new (the pick's area; picks are > 2*roi apart so their sheds share
       // no bit, only, possibly, an edge word -- hence the atomic)
@@ -743,7 +769,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         }
         const int i = rng_lo[lo] + item - (lo > 0 ? rng_end[lo - 1] : 0);
         const unsigned int bit = 1u << (i & 31);
-        if (!U[i] && !(atomicOr(&seen[i >> 5], bit) & bit)) {
+        if (!(U[i] & 1) && !(atomicOr(&seen[i >> 5], bit) & bit)) {
           const uint32_t ps = P[i];
           const int cx = (int)(ps >> 16), cy = (int)(ps & 0xffffu);
           int m = 0;
@@ -752,7 +778,11 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
             const int ey = cy - (int)(bw_pos[j] & 0xffffu);
             m += ex * ex + ey * ey <= r2;
           }
-          if (m) {
+          if (m && hybrid && ((U[i] & 2) || T[i] < tau)) {
+            my_u += m;  // keep the old area as a bound
+            U[i] |= 2;
+            dirty[((cx / a.cs) * Q + (cy / a.cs) % Q) / G] = 1;
+          } else if (m) {
             int acc = m;
             if (!stop && a.intervis) {  // cpp:452-457 (one pick here)
               uint64_t word = a.cum[(size_t)cx * cw + (cy >> 6)];
@@ -802,6 +832,50 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     for (int i = tid; i < (cnt + 31) / 32; i += kLoopThreads) seen[i] = 0;
     if (stop) break;
     __syncthreads();
+    if (hybrid) {
+      // re-evaluate the stale candidates whose bound reaches tau against cum
+      // (complete: the union ended with a grid barrier); a group of waves
+      // shares a candidate when there are few
+      Winner W0{};
+      for (;;) {
+        if (tid == 0) list_n = 0;
+        __syncthreads();
+        for (int i = tid; i < cnt; i += kLoopThreads)
+          if ((U[i] & 3) == 2 && T[i] >= tau) {
+            const int at = atomicAdd(&list_n, 1);
+            if (at < kMaxList) list_i[at] = i;
+          }
+        __syncthreads();
+        const int want = list_n;
+        const int lcnt = want < kMaxList ? want : kMaxList;
+        if (lcnt == 0) break;
+        for (int e = tid; e < lcnt; e += kLoopThreads) list_acc[e] = 0;
+        __syncthreads();
+        const int gw = lcnt == 1 ? 16 : lcnt == 2 ? 8 : lcnt <= 4 ? 4 : lcnt <= 8 ? 2 : 1;
+        const int ngroups = kLoopWaves / gw;
+        const int grp = wid / gw, gl = (wid - grp * gw) * kWave + lane;
+        for (int e = grp; e < lcnt; e += ngroups) {
+          const int i = list_i[e];
+          const uint32_t ps = P[i];
+          const int cx = (int)(ps >> 16), cy = (int)(ps & 0xffffu);
+          const uint64_t *sh = a.sheds + (size_t)C[i] * S;
+          const int s = wave_sum_i32(lane_new_bits(sh, cx, cy, a, W0, gl, gw * kWave, false));
+          if (lane == 0) atomicAdd(&list_acc[e], s);
+        }
+        __syncthreads();
+        for (int e = tid; e < lcnt; e += kLoopThreads) {
+          const int i = list_i[e];
+          T[i] = list_acc[e];
+          U[i] = 0;
+          const uint32_t ps = P[i];
+          dirty[((int)(ps >> 16) / a.cs * Q + ((int)(ps & 0xffffu) / a.cs) % Q) / G] = 1;
+        }
+        __syncthreads();
+        if (want <= kMaxList) break;
+      }
+      // the update walk of the next round starts from an empty list
+      if (tid == 0) list_n = 0;
+    }
     for (int ul = wid; ul < upw; ul += kLoopWaves)
       if (dirty[ul]) unit_top2(ul);
     __syncthreads();
@@ -867,10 +941,28 @@ struct Plan {
       o_stk, o_seen;
 };
 
-int env_q() {
+// Column classes per cell row.  More classes spread a pick's update window
+// over more workgroups but lengthen the selection scan; three pay only when
+// a pick's expected update list pi*(2roi)^2 * nsheds/n^2 reaches 3000
+// (measured).  SITING_SITE_Q overrides.
+int choose_q(int n, int roi, int nsheds) {
   const char *e = std::getenv("SITING_SITE_Q");
-  int q = e ? std::atoi(e) : 2;
-  return q < 1 ? 1 : (q > 8 ? 8 : q);
+  if (e && *e) {
+    const int q = std::atoi(e);
+    return q < 1 ? 1 : (q > 8 ? 8 : q);
+  }
+  const double list = 3.14159265358979 * 4.0 * (double)roi * roi * (double)nsheds /
+                      ((double)n * (double)n);
+  return list >= 3000.0 ? 3 : 2;
+}
+
+// Update candidates whose area is below tau8/8 of the round's largest are
+// not decremented but kept as stale bounds (SITING_SITE_TAU8 overrides; 0
+// turns it off).
+int tau8() {
+  const char *e = std::getenv("SITING_SITE_TAU8");
+  const int t = e && *e ? std::atoi(e) : 7;
+  return t < 0 ? 0 : (t > 8 ? 8 : t);
 }
 
 // Cell edge: about 30 cell rows across the 4*roi update window, and at most
@@ -888,7 +980,7 @@ Plan make_plan(int n, int roi, int nsheds, int groups_cap, int fixed_groups = 0)
   Plan p{};
   p.cs = cell_edge(n, roi);
   p.ncr = (n + p.cs - 1) / p.cs;
-  p.Q = env_q();
+  p.Q = choose_q(n, roi, nsheds);
   if (p.Q > p.ncr) p.Q = p.ncr;
   p.nunits = p.ncr * p.Q;
   const int touched = ((4 * roi) / p.cs + 2) * p.Q;
@@ -1050,6 +1142,7 @@ siting_status site_make_args(const Geometry &g, const int32_t *obs,
   a.o_t = p.o_t;
   a.o_used = p.o_used;
   a.o_seen = p.o_seen;
+  a.tau8 = tau8();
   a.npts = (double)n * (double)n;
   *out = a;
   if (lds) *lds = p.lds;
