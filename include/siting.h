@@ -202,6 +202,8 @@ siting_status siting_batch_run(siting_batch *b);
 siting_status siting_batch_sync(siting_batch *b);
 /* Device time of the last run, from events around it (after sync). */
 siting_status siting_batch_elapsed_ms(siting_batch *b, float *ms);
+/* Device time of the last run's SITE work-queue launch alone (after sync). */
+siting_status siting_batch_site_ms(siting_batch *b, float *ms);
 /* Borrowed handle of job i, for downloads and stats (valid until destroy). */
 siting_status siting_batch_job(siting_batch *b, int job, siting_job **out);
 

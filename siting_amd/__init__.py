@@ -117,6 +117,7 @@ _SIGS = {
     "siting_batch_run": (_I, [_VP]),
     "siting_batch_sync": (_I, [_VP]),
     "siting_batch_elapsed_ms": (_I, [_VP, ctypes.POINTER(ctypes.c_float)]),
+    "siting_batch_site_ms": (_I, [_VP, ctypes.POINTER(ctypes.c_float)]),
     "siting_batch_job": (_I, [_VP, _I, ctypes.POINTER(_VP)]),
     "siting_server_create": (_I, [_I, _VP, ctypes.POINTER(_VP)]),
     "siting_server_destroy": (None, [_VP]),
@@ -389,6 +390,12 @@ class Batch:
     def elapsed_ms(self) -> float:
         ms = ctypes.c_float()
         _check(lib().siting_batch_elapsed_ms(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def site_ms(self) -> float:
+        """Device time of the last run's SITE work-queue launch."""
+        ms = ctypes.c_float()
+        _check(lib().siting_batch_site_ms(self._h, ctypes.byref(ms)))
         return ms.value
 
     def job(self, i) -> Job:

@@ -88,31 +88,66 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
     const int len = along_y ? ady : adx;
     const int dir = major > 0 ? 1 : -1;
     const float slope = (float)minor / (float)major;  // siting.cpp:323
-    float horizon = -99999.0f;                          // siting.cpp:325
-    for (int a = 1; a <= len; a++) {
+    // cell of step a (siting.cpp:330): the minor offset is the rounded float
+    // product, as the reference computes it
+    auto cell = [&](int a, int &qx, int &qy) {
       const int i = dir * a;
-      const int off = round_half_away((float)i * slope);  // siting.cpp:330
-      const int qx = along_y ? off : i;
-      const int qy = along_y ? i : off;
-      const int px = ox + qx, py = oy + qy;
-      if (px < 0 || px >= n || py < 0 || py >= n) break;  // siting.cpp:333
-      if (qx * qx + qy * qy > rr) break;                  // siting.cpp:334
-      const int pel = (int)elevs[px * n + py];
-      const float fa = (float)a;
-      // siting.cpp:340, (float)(pel - oalt) / (float)a.  With |pel - oalt| <
-      // 2^21 and a < 2^14 the quotient is never closer than 2^-39 (relative)
-      // to a float rounding midpoint, and never on one, while the double
-      // product below is within 2^-52 of it: rounding it to float gives the
-      // correctly rounded fp32 quotient, bit for bit.
+      const int off = round_half_away((float)i * slope);
+      qx = along_y ? off : i;
+      qy = along_y ? i : off;
+    };
+    // The reference walks a = 1..len and stops at the first cell outside the
+    // terrain or the roi disk (siting.cpp:333-334).  Both tests are monotone
+    // in a: i moves away from the observer, and so does the minor offset (a
+    // float product of a fixed slope, rounded half away from zero, is
+    // monotone in i).  The steps taken are therefore 1..last, with last
+    // found by bisection; the walk below then needs no exit tests.
+    int last = 0;
+    {
+      int hi = len + 1;  // cell(last) inside (a = 0 is the observer), cell(hi) not
+      while (hi - last > 1) {
+        const int mid = (last + hi) >> 1;
+        int qx, qy;
+        cell(mid, qx, qy);
+        const int px = ox + qx, py = oy + qy;
+        const bool in = px >= 0 && px < n && py >= 0 && py < n && qx * qx + qy * qy <= rr;
+        if (in) last = mid; else hi = mid;
+      }
+    }
+    float horizon = -99999.0f;  // siting.cpp:325
+    // siting.cpp:337-344 for one step.  The point slope is siting.cpp:340,
+    // (float)(pel - oalt) / (float)a: with |pel - oalt| < 2^21 and a < 2^14
+    // the quotient is never closer than 2^-39 (relative) to a float rounding
+    // midpoint, and never on one, while the double product below is within
+    // 2^-52 of it, so rounding it to float gives the correctly rounded fp32
+    // quotient, bit for bit.
+    auto step = [&](int a, int pel, int qx, int qy) {
       const float sl = (float)((double)(pel - oalt) * rd[a]);
       if (horizon < sl) horizon = sl;
-      const float rise = horizon * fa;
+      const float rise = horizon * (float)a;
       const float halt = foalt + rise;  // siting.cpp:342
       if ((float)(pel + tht) >= halt) {
         const int row = qx + roi, col = qy + roi;
         atomicOr((unsigned long long *)&bits[row * nwpr + (col >> 6)],
                  1ull << (63 - (col & 63)));
       }
+    };
+    // four steps per pass: their DEM loads are issued before any is used
+    int a = 1;
+    for (; a + 3 <= last; a += 4) {
+      int qx[4], qy[4], pel[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        cell(a + u, qx[u], qy[u]);
+        pel[u] = (int)elevs[(ox + qx[u]) * n + (oy + qy[u])];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) step(a + u, pel[u], qx[u], qy[u]);
+    }
+    for (; a <= last; a++) {
+      int qx, qy;
+      cell(a, qx, qy);
+      step(a, (int)elevs[(ox + qx) * n + (oy + qy)], qx, qy);
     }
   }
 

@@ -48,3 +48,63 @@ def test_float_index_split_exact():
         inv = np.float32(1.0) / np.float32(nk)
         q = ((it.astype(np.float32) + np.float32(0.5)) * inv).astype(np.int64)
         assert np.array_equal(q, it // nk), nk
+
+
+def _ray_cells(roi, ray):
+    """Perimeter target of ray `ray` and the ray's (major, minor, along_y, len,
+    dir, float32 slope), as VIEWSHED enumerates them (siting.cpp:292-323)."""
+    twor = 2 * roi
+    side, j = divmod(ray, twor)
+    if side == 0:
+        dx, dy = -roi + j, -roi
+    elif side == 1:
+        dx, dy = -roi + 1 + j, roi
+    elif side == 2:
+        dx, dy = -roi, -roi + 1 + j
+    else:
+        dx, dy = roi, -roi + j
+    along_y = abs(dx) < abs(dy)
+    major, minor = (dy, dx) if along_y else (dx, dy)
+    slope = np.float32(minor) / np.float32(major)
+    return along_y, abs(major), (1 if major > 0 else -1), slope
+
+
+def _offsets(dirn, slope, a):
+    x = (np.float32(dirn) * a.astype(np.float32)) * slope
+    x = x.astype(np.float32)
+    t = np.trunc(x).astype(np.float32)
+    f = (x - t).astype(np.float32)
+    return t.astype(np.int64) + (f >= 0.5) - (f <= -0.5)
+
+
+def test_viewshed_ray_prefix_by_bisection():
+    """VIEWSHED finds the last step of a ray by bisection instead of testing
+    every step (siting.cpp:333-334 break at the first cell outside the
+    terrain or the roi disk).  That needs the valid steps to be a prefix:
+    checked against the step-by-step walk for every ray of several roi, with
+    observers at corners, edges and inside small terrains."""
+    for roi in (1, 2, 3, 5, 7, 13, 31, 64, 100, 150):
+        rays = range(8 * roi)
+        for n in (1, 2, 7, 40, 301):
+            obs = {(0, 0), (0, n - 1), (n - 1, 0), (n - 1, n - 1), (n // 2, n // 2),
+                   (0, n // 2), (n // 3, n - 1), (min(n - 1, roi), min(n - 1, 2))}
+            for ox, oy in obs:
+                for ray in rays:
+                    along_y, length, dirn, slope = _ray_cells(roi, ray)
+                    a = np.arange(1, length + 1, dtype=np.int64)
+                    off = _offsets(dirn, slope, a)
+                    i = dirn * a
+                    qx, qy = (off, i) if along_y else (i, off)
+                    px, py = ox + qx, oy + qy
+                    ok = (px >= 0) & (px < n) & (py >= 0) & (py < n) & (qx * qx + qy * qy <= roi * roi)
+                    walk = int(np.argmin(ok)) if not ok.all() else length
+                    # bisection as in the kernel: cell(last) inside, cell(hi) not
+                    last, hi = 0, length + 1
+                    while hi - last > 1:
+                        mid = (last + hi) // 2
+                        if ok[mid - 1]:
+                            last = mid
+                        else:
+                            hi = mid
+                    assert last == walk, (roi, n, ox, oy, ray)
+                    assert ok[:last].all()
