@@ -132,6 +132,22 @@ def algorithmic_bytes(n, d, st):
     }
 
 
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "site_loop_queue_uneven_batch64_pmc.md")
+
+
+def pmc_traffic():
+    """Bytes per launch of the batch's SITE work-queue kernel, as measured by
+    the two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE, scaled by a plain
+    copy) recorded in the profile summary; None when it is absent."""
+    try:
+        for line in open(PMC_SUMMARY):
+            if line.startswith("| traffic per launch |"):
+                return int(float(line.split("|")[2].split()[0]) * 1e9)
+    except OSError:
+        pass
+    return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -258,7 +274,7 @@ def main():
                 "bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
                 "bytes_per_launch": site_bytes, "ms_per_launch": round(site_ms_avg, 3),
-                "traffic": None}
+                "traffic": pmc_traffic()}
     log(f"single job: {1000 * t_single:.2f} ms/step, stages {stage_ms}")
 
     serve = None

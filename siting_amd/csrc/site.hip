@@ -548,15 +548,30 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       // lane-local best over units lane, lane+64, ...: the unit's first key,
       // or its second once the first is taken this round (or the taken
       // second, as an upper bound of the unknown third)
+      // (units in chunks of 8 per lane, so that each chunk's LDS reads are
+      // in flight together)
       auto lane_best = [&](uint64_t &lv, int &lu) {
         lv = 0;
         lu = -1;
-        for (int u = lane; u < nunits; u += kWave) {
-          const uint64_t v = sk[2 * u + (stk[u] == 0 ? 0 : 1)];
-          if (v > lv) {
-            lv = v;
-            lu = u;
+        for (int u0 = lane; u0 < nunits; u0 += 8 * kWave) {
+          int t[8];
+#pragma unroll
+          for (int j = 0; j < 8; j++) {
+            const int u = u0 + j * kWave;
+            t[j] = u < nunits ? stk[u] : 0;
           }
+          uint64_t v[8];
+#pragma unroll
+          for (int j = 0; j < 8; j++) {
+            const int u = u0 + j * kWave;
+            v[j] = u < nunits ? sk[2 * u + (t[j] == 0 ? 0 : 1)] : 0;
+          }
+#pragma unroll
+          for (int j = 0; j < 8; j++)
+            if (v[j] > lv) {
+              lv = v[j];
+              lu = u0 + j * kWave;
+            }
         }
       };
       uint64_t lv;
