@@ -94,7 +94,7 @@ Layout layout_for(const siting_params &p, const siting_derived &d) {
   put(L.state, sizeof(SiteState));
   put(L.recip, ((size_t)p.roi + 1) * 4);
   put(L.jump, 4 * (size_t)p.ntests * 4);
-  put(L.rdiv, ((size_t)p.roi + 1) * 8);
+  put(L.rdiv, rdiv_entries(p.roi) * 8);
   put(L.scratch, site_scratch_bytes(p.nrows, p.roi, d.nwanted));
   L.total = off;
   return L;
@@ -225,7 +225,7 @@ siting_status job_create_in(int device, const siting_params *p, void *mem,
   j->rdiv = (double *)(b + L.rdiv);
   std::vector<float> rec((size_t)p->roi + 1);
   std::vector<uint32_t> jmp(4 * (size_t)p->ntests);
-  std::vector<double> rdv((size_t)p->roi + 1);
+  std::vector<double> rdv(rdiv_entries(p->roi), 0.0);
   fill_recip_table(p->roi, rec.data());
   fill_jump_table(p->ntests, jmp.data());
   fill_rdiv_table(p->roi, rdv.data());
@@ -514,8 +514,8 @@ siting_status siting_calc_sheds(int nrows, const uint16_t *elevs, int roi, int o
   CHECK(dalloc(de, n2 * 2));
   CHECK(dalloc(dobs, (size_t)nsheds * 4));
   CHECK(dalloc(dsh, sbytes));
-  CHECK(dalloc(drd, ((size_t)roi + 1) * 8));
-  std::vector<double> rdv((size_t)roi + 1);
+  CHECK(dalloc(drd, rdiv_entries(roi) * 8));
+  std::vector<double> rdv(rdiv_entries(roi), 0.0);
   fill_rdiv_table(roi, rdv.data());
   SITING_HIP(hipMemcpyAsync(de.p, elevs, n2 * 2, hipMemcpyHostToDevice, s));
   SITING_HIP(hipMemcpyAsync(dobs.p, obs, (size_t)nsheds * 4, hipMemcpyHostToDevice, s));

@@ -116,7 +116,11 @@ siting_status launch_vix(const Geometry &g, const uint16_t *elevs, uint8_t *vix,
                          const VixTables &t, hipStream_t s);
 siting_status launch_findmax(const Geometry &g, uint8_t *vix, int32_t *obs,
                              hipStream_t s);
-// rdiv: 1.0/a in double for a = 0..roi (entry 0 unused)
+// rdiv: 1.0/a in double for a = 0..roi (entry 0 unused), then zeros up to
+// rdiv_entries(roi): the kernel reads the table with scalar loads four steps
+// at a time, and a scalar load under a lane-divergent condition may run for
+// a step that no lane takes (at most three past roi).
+inline size_t rdiv_entries(int roi) { return (size_t)roi + 4; }
 siting_status launch_viewshed(const Geometry &g, const uint16_t *elevs,
                               const int32_t *obs, uint64_t *sheds,
                               const double *rdiv, hipStream_t s);

@@ -28,29 +28,69 @@ namespace siting {
 namespace {
 
 constexpr int kMaxThreads = 1024;
-constexpr int kLdsWords = 8192;  // 64 KiB bitset budget
+constexpr int kLdsWords = 8192;          // 64 KiB bitset budget
+constexpr size_t kSplitBytes = 48 * 1024;  // budget of the two split bitsets
 
-template <bool kInLds, bool kRdLds>
+// Where a workgroup keeps its observer's bitset while the rays run:
+//   kGlobal  straight in HBM (device atomics), for bitsets above kLdsWords
+//   kLds     one LDS bitset in the output layout
+//   kSplit   two LDS bitsets with an odd row stride ps: rows for the rays
+//            whose major axis is y, and a transposed one (a run of words per
+//            column) for the rays whose major axis is x.  At step a every
+//            x-major ray lies on one of two rows, so in the output layout
+//            their lanes would all OR into the same few words; transposed,
+//            each lane has a word of its own.  The transposed bitset is
+//            turned back, 64x64 bits per wavefront with ballots, when the
+//            output is written.
+enum { kGlobal = 0, kLds = 1, kSplit = 2 };
+
+__device__ __forceinline__ int split_stride(int nwpr) { return nwpr | 1; }
+
+// 64x64 bit transpose across a wavefront: lane l holds row l (bit c = column
+// c); returns lane l's column l as a row, i.e. result bit c = input bit l of
+// lane c.  Six butterfly stages swap the off-diagonal j x j quadrants of
+// every 2j x 2j block.
+__device__ __forceinline__ uint64_t wave_transpose64(uint64_t x, int lane) {
+  const uint64_t m[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                         0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+  for (int s = 0; s < 6; s++) {
+    const int j = 32 >> s;
+    const uint64_t t = __shfl_xor(x, j, kWave);
+    x = (lane & j) ? ((x & ~m[s]) | ((t & ~m[s]) >> j)) : ((x & m[s]) | ((t & m[s]) << j));
+  }
+  return x;
+}
+
+__device__ __forceinline__ int wave_max_i32(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(v, off, kWave);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+template <int kMode>
 __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
     const uint16_t *__restrict__ elevs, const int32_t *__restrict__ obs,
-    uint64_t *__restrict__ sheds, const double *__restrict__ rdiv, int n, int roi,
+    uint64_t *__restrict__ sheds, const double *__restrict__ rd, int n, int roi,
     int oht, int tht, int nwpr, int nsheds, int per_xcd) {
   const int nthreads = blockDim.x;
-  extern __shared__ uint64_t lds_dyn[];  // [roi + 1] reciprocals?, bitset?
+  extern __shared__ uint64_t lds_dyn[];  // the bitset(s) (kLds, kSplit)
   const int b = blockIdx.x;
   const int k = (b & 7) * per_xcd + (b >> 3);
   if (k >= nsheds) return;
   const int nr = 2 * roi + 1;
   const int words = nr * nwpr;
+  const int ps = split_stride(nwpr);
   uint64_t *out = sheds + (size_t)k * words;
-  double *rd_l = (double *)lds_dyn;
-  const double *rd = kRdLds ? rd_l : rdiv;
-  uint64_t *bits = kInLds ? lds_dyn + (kRdLds ? roi + 1 : 0) : out;
+  uint64_t *bits = kMode == kGlobal ? out : lds_dyn;
+  uint64_t *bits_t = lds_dyn + (size_t)nr * ps;  // kSplit: transposed bitset
 
-  if (kRdLds)
-    for (int i = threadIdx.x; i <= roi; i += nthreads) rd_l[i] = rdiv[i];
-  for (int i = threadIdx.x; i < words; i += nthreads) bits[i] = 0;
-  if (!kInLds) __threadfence();
+  const int clear = kMode == kSplit ? 2 * nr * ps : words;
+  for (int i = threadIdx.x; i < clear; i += nthreads) bits[i] = 0;
+  if (kMode == kGlobal) __threadfence();
   __syncthreads();
 
   const int o = obs[k];
@@ -61,10 +101,16 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
   const int twor = 2 * roi;
 
   if (threadIdx.x == 0)  // observer sees itself, siting.cpp:279
-    atomicOr((unsigned long long *)&bits[roi * nwpr + (roi >> 6)],
+    atomicOr((unsigned long long *)&bits[roi * (kMode == kSplit ? ps : nwpr) + (roi >> 6)],
              1ull << (63 - (roi & 63)));
 
-  for (int ray = threadIdx.x; ray < 8 * roi; ray += nthreads) {
+  // Rays are dealt in passes of nthreads; every lane of a pass takes part
+  // (lanes past the last ray walk no step), so that each wavefront walks
+  // its rays' steps together: the step index is then uniform across the
+  // wavefront and 1.0/step comes from the table through scalar loads.
+  for (int base = 0; base < 8 * roi; base += nthreads) {
+    const bool live = base + (int)threadIdx.x < 8 * roi;
+    const int ray = live ? base + (int)threadIdx.x : 0;
     const int side = ray / twor;
     const int j = ray - side * twor;
     int dx, dy;
@@ -103,7 +149,7 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
     // monotone in i).  The steps taken are therefore 1..last, with last
     // found by bisection; the walk below then needs no exit tests.
     int last = 0;
-    {
+    if (live) {
       int hi = len + 1;  // cell(last) inside (a = 0 is the observer), cell(hi) not
       while (hi - last > 1) {
         const int mid = (last + hi) >> 1;
@@ -128,32 +174,52 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
       const float halt = foalt + rise;  // siting.cpp:342
       if ((float)(pel + tht) >= halt) {
         const int row = qx + roi, col = qy + roi;
-        atomicOr((unsigned long long *)&bits[row * nwpr + (col >> 6)],
-                 1ull << (63 - (col & 63)));
+        if (kMode == kSplit) {
+          const int w = along_y ? row * ps + (col >> 6) : nr * ps + col * ps + (row >> 6);
+          const int bit = along_y ? (col & 63) : (row & 63);
+          atomicOr((unsigned long long *)&bits[w], 1ull << (63 - bit));
+        } else {
+          atomicOr((unsigned long long *)&bits[row * nwpr + (col >> 6)],
+                   1ull << (63 - (col & 63)));
+        }
       }
     };
-    // four steps per pass: their DEM loads are issued before any is used
-    int a = 1;
-    for (; a + 3 <= last; a += 4) {
+    // four steps at a time, their DEM loads issued before any is used; the
+    // wavefront runs to its longest ray, each lane stepping up to its own
+    const int wlast = __builtin_amdgcn_readfirstlane(wave_max_i32(last));
+    for (int a = 1; a <= wlast; a += 4) {
       int qx[4], qy[4], pel[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         cell(a + u, qx[u], qy[u]);
-        pel[u] = (int)elevs[(ox + qx[u]) * n + (oy + qy[u])];
+        pel[u] = a + u <= last ? (int)elevs[(ox + qx[u]) * n + (oy + qy[u])] : 0;
       }
 #pragma unroll
-      for (int u = 0; u < 4; u++) step(a + u, pel[u], qx[u], qy[u]);
-    }
-    for (; a <= last; a++) {
-      int qx, qy;
-      cell(a, qx, qy);
-      step(a, (int)elevs[(ox + qx) * n + (oy + qy)], qx, qy);
+      for (int u = 0; u < 4; u++)
+        if (a + u <= last) step(a + u, pel[u], qx[u], qy[u]);
     }
   }
 
-  if (kInLds) {
+  if (kMode == kLds) {
     __syncthreads();
     for (int i = threadIdx.x; i < words; i += nthreads) out[i] = bits[i];
+  } else if (kMode == kSplit) {
+    // block (R, W): rows 64R..64R+63, output word W (columns 64W..64W+63).
+    // Lane l loads column 64W+l's transposed word for those rows (bit 63-r
+    // for row 64R+r).  Bit-reversed, that is row l of a matrix whose bit r
+    // is row 64R+r; transposed, lane l holds row 64R+l with bit c for
+    // column 64W+c, which bit-reversed is the output word (bit 63-c).
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave, nwaves = nthreads / kWave;
+    for (int blk = wid; blk < nwpr * nwpr; blk += nwaves) {
+      const int R = blk / nwpr, W = blk - R * nwpr;
+      const int c = 64 * W + lane;
+      const uint64_t tw = c < nr ? bits_t[c * ps + R] : 0;
+      const uint64_t mine = __brevll(wave_transpose64(__brevll(tw), lane));
+      const int row = 64 * R + lane;
+      if (row < nr) out[row * nwpr + W] = bits[row * ps + W] | mine;
+    }
   }
 }
 
@@ -178,18 +244,17 @@ siting_status launch_viewshed(const Geometry &g, const uint16_t *elevs,
     const int t = std::atoi(e);
     if (t >= kWave && t <= kMaxThreads && t % kWave == 0) threads = t;
   }
-  const bool rd_lds = g.p.roi < 4096;  // 32 KiB table
-  const size_t rd_bytes = rd_lds ? sizeof(double) * ((size_t)g.p.roi + 1) : 0;
-  const bool in_lds = words <= kLdsWords;
-  const size_t lds = rd_bytes + (in_lds ? sizeof(uint64_t) * words : 0);
-#define SITING_VS(A, B)                                                             \
-  hipLaunchKernelGGL((viewshed_kernel<A, B>), dim3(8 * per_xcd), dim3(threads), lds, \
-                     s, elevs, obs, sheds, rdiv, g.p.nrows, g.p.roi, g.p.oht, g.p.tht, \
+  const size_t split_bytes = 2 * sizeof(uint64_t) * g.d.nr * (size_t)(g.d.nwpr | 1);
+  const int mode = split_bytes <= kSplitBytes ? kSplit : words <= kLdsWords ? kLds : kGlobal;
+  const size_t lds = mode == kSplit ? split_bytes
+                     : mode == kLds ? sizeof(uint64_t) * words : 0;
+#define SITING_VS(A)                                                               \
+  hipLaunchKernelGGL((viewshed_kernel<A>), dim3(8 * per_xcd), dim3(threads), lds, s, \
+                     elevs, obs, sheds, rdiv, g.p.nrows, g.p.roi, g.p.oht, g.p.tht,   \
                      g.d.nwpr, nsheds, per_xcd)
-  if (in_lds && rd_lds) SITING_VS(true, true);
-  else if (in_lds) SITING_VS(true, false);
-  else if (rd_lds) SITING_VS(false, true);
-  else SITING_VS(false, false);
+  if (mode == kSplit) SITING_VS(kSplit);
+  else if (mode == kLds) SITING_VS(kLds);
+  else SITING_VS(kGlobal);
 #undef SITING_VS
   SITING_HIP(hipGetLastError());
   return SITING_OK;

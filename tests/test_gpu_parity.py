@@ -28,6 +28,8 @@ CASES = [
     (97, 5, 0, 1, 7, 200, 0, 0.7, 20000, 7),          # one target, ragged blocks
     (33, 3, 100, 8, 3, 1000, 1, 0.6, 30000, 8),       # tiny blocks, nwpb 9
     (150, 64, 100, 20, 25, 60, 0, 0.8, 20000, 9),     # nwpr exactly 3 words
+    (120, 260, 100, 8, 40, 9, 0, 0.7, 20000, 10),     # bitset in one LDS array
+    (130, 400, 50, 8, 43, 9, 1, 0.7, 20000, 11),      # bitset in HBM
 ]
 
 
