@@ -14,6 +14,8 @@
 // Tiles are assigned XCD-major: workgroup b runs on XCD b%8, and each XCD
 // gets one contiguous band of tiles so the DEM window it gathers from stays
 // in its own L2.
+#include <cstdlib>
+
 #include "common.hpp"
 
 namespace siting {
@@ -31,6 +33,7 @@ __device__ __forceinline__ int target_offset(float span, uint32_t r) {
   return (int)q;
 }
 
+template <int kGroup>
 __global__ __launch_bounds__(kThreads) void vix_kernel(
     const uint16_t *__restrict__ elevs, uint8_t *__restrict__ vix,
     const float *__restrict__ recip, const uint32_t *__restrict__ jump,
@@ -62,6 +65,7 @@ __global__ __launch_bounds__(kThreads) void vix_kernel(
   for (int q = threadIdx.x; q < total; q += kThreads) {
     const int px = tx0 + (lp >> 4);
     const int py = ty0 + (lp & 15);
+    bool vis = false;
     if (px < n && py < n) {
       const uint32_t p = (uint32_t)(px * n + py);
       const uint32_t r1 = (jl[4 * k] * p + jl[4 * k + 1]) & 0x7fffffffu;
@@ -73,10 +77,7 @@ __global__ __launch_bounds__(kThreads) void vix_kernel(
         tx = px + ddx / 3;
         ty = py + ddy / 3;
       }
-      bool vis = false;
       if (tx >= 0 && tx < n && ty >= 0 && ty < n) {
-        const int oz = (int)elevs[p] + oht;
-        const int tz = (int)elevs[tx * n + ty] + tht;
         const int dx = tx - px, dy = ty - py;
         const int adx = dx < 0 ? -dx : dx, ady = dy < 0 ? -dy : dy;
         if (adx <= 1 && ady <= 1) {
@@ -89,26 +90,59 @@ __global__ __launch_bounds__(kThreads) void vix_kernel(
           float rec = recip[len];
           if (major < 0) rec = -rec;  // (float)(1.0/-d) == -(float)(1.0/d)
           const float slope = (float)minor * rec;
-          const float zs = (float)(tz - oz) * rec;
           const int dir = major > 0 ? 1 : -1;
+          // The reference samples id = dir*1, dir*2, dir*4, ... while |id| <
+          // len and stops at the first sample above the sight line, so the
+          // target is visible iff no sample is above it: the samples can be
+          // read in any order.  kGroup at a time, the first group in flight
+          // together with the observer and target elevations.
+          const int oz_raw = (int)elevs[p];
+          const int tz_raw = (int)elevs[tx * n + ty];
+          float zz[kGroup];
+          auto load_group = [&](int s0) {
+#pragma unroll
+            for (int u = 0; u < kGroup; u++) {
+              const int sv = s0 << u;
+              const int id = dir * sv;
+              const int off = round_half_away(slope * (float)id);
+              const int sx = alx ? px + id : px + off;
+              const int sy = alx ? py + off : py + id;
+              zz[u] = sv < len ? (float)elevs[sx * n + sy] : 0.0f;
+            }
+          };
+          load_group(1);
+          const int oz = oz_raw + oht;
+          const int tz = tz_raw + tht;
+          const float zs = (float)(tz - oz) * rec;
           const float foz = (float)oz;
           vis = true;
-          for (int s = 1; s < len; s <<= 1) {
-            const int id = dir * s;
-            const float fid = (float)id;
-            const int off = round_half_away(slope * fid);
-            const int sx = alx ? px + id : px + off;
-            const int sy = alx ? py + off : py + id;
-            const float z = (float)elevs[sx * n + sy];
-            const float lim = foz + fid * zs;
-            if (z > lim) {
-              vis = false;
-              break;
+          for (int s0 = 1;;) {
+#pragma unroll
+            for (int u = 0; u < kGroup; u++) {
+              const int sv = s0 << u;
+              const float fid = (float)(dir * sv);
+              const float lim = foz + fid * zs;
+              if (sv < len && zz[u] > lim) vis = false;
             }
+            s0 <<= kGroup;
+            if (!vis || s0 >= len) break;
+            load_group(s0);
           }
         }
       }
-      if (vis) atomicAdd(&count[lp], 1);
+    }
+    // visible pairs per point: the wavefront's pairs are consecutive, so a
+    // point's pairs are a run of lanes; one atomic per run
+    {
+      const uint64_t vm = __ballot(vis);
+      const int lane = threadIdx.x & (kWave - 1);
+      const int lp_first = __builtin_amdgcn_readfirstlane(lp);
+      const int lp_last = __shfl(lp, kWave - 1, kWave);
+      for (int j = lp_first; j <= lp_last; j++) {
+        const uint64_t run = __ballot(lp == j);
+        const int c = __popcll(vm & run);
+        if (c && lane == (int)__builtin_ctzll(run)) atomicAdd(&count[j], c);
+      }
     }
     lp += step_lp;
     k += step_k;
@@ -154,9 +188,17 @@ siting_status launch_vix(const Geometry &g, const uint16_t *elevs, uint8_t *vix,
   const int ntiles = side * side;
   const int per_xcd = (ntiles + 7) / 8;
   const size_t lds = sizeof(uint32_t) * (4 * (size_t)g.p.ntests + kThreads);
-  hipLaunchKernelGGL(vix_kernel, dim3(8 * per_xcd), dim3(kThreads), lds, s,
-                     elevs, vix, t.recip, t.jump, n, g.p.roi, g.p.oht, g.p.tht,
-                     g.p.ntests, side, per_xcd);
+  // samples read per dependent step (tuning override SITING_VIX_GROUP)
+  int group = 1;
+  if (const char *e = std::getenv("SITING_VIX_GROUP")) group = std::atoi(e);
+#define SITING_VIX(G)                                                              \
+  hipLaunchKernelGGL(vix_kernel<G>, dim3(8 * per_xcd), dim3(kThreads), lds, s, elevs, \
+                     vix, t.recip, t.jump, n, g.p.roi, g.p.oht, g.p.tht, g.p.ntests, \
+                     side, per_xcd)
+  if (group == 4) SITING_VIX(4);
+  else if (group == 2) SITING_VIX(2);
+  else SITING_VIX(1);
+#undef SITING_VIX
   SITING_HIP(hipGetLastError());
   return SITING_OK;
 }
