@@ -52,9 +52,8 @@
 // loop.  Each key travels with its candidate's position.
 //
 // Barriers are plain atomics on a counter, so every workgroup of a loop must
-// be resident at once: the host caps G at half the CUs (one 1024-thread
-// workgroup per CU fits the loop's LDS), and the work-queue launch sizes its
-// slots the same way.
+// be resident at once: the host caps G at half the CUs (one workgroup per
+// CU), and the work-queue launch sizes its slots the same way.
 #include <cstdlib>
 
 #include "common.hpp"
@@ -63,9 +62,13 @@ namespace siting {
 
 namespace {
 
+constexpr int kScanThreads = 1024;  // the bucketing scan kernels
+// The loop's workgroups are 1024 threads (four wavefronts per SIMD, so at
+// most 128 VGPRs) with up to 132 KiB of dynamic LDS: one per CU.  (512-thread
+// groups that leave room for the other stages were measured 6-35 % slower.)
 constexpr int kLoopThreads = 1024;
 constexpr int kLoopWaves = kLoopThreads / kWave;
-constexpr int kMaxList = 1024;
+constexpr int kMaxList = kLoopThreads;  // list windows: one item per thread
 constexpr size_t kLdsBudget = 132 * 1024;  // dynamic LDS of the loop kernel
 constexpr int kMaxGroups = 128;
 constexpr int kMaxBatch = kSiteMaxBatch;
@@ -97,8 +100,10 @@ __global__ void site_bucket_count(SiteArgs a) {
   atomicAdd(&a.bcount[bucket_of(a, o / a.n, o % a.n)], 1);
 }
 
-// inclusive scan of one value per thread over a 1024-thread block
+// inclusive scan of one value per thread over a kThreads-thread block
+template <int kThreads>
 __device__ int block_inclusive_scan(int v, int *wsum) {
+  constexpr int kWaves = kThreads / kWave;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
@@ -108,47 +113,47 @@ __device__ int block_inclusive_scan(int v, int *wsum) {
   if (lane == 63) wsum[wid] = v;
   __syncthreads();
   if (wid == 0) {
-    int s = lane < kLoopWaves ? wsum[lane] : 0;
+    int s = lane < kWaves ? wsum[lane] : 0;
 #pragma unroll
-    for (int off = 1; off < 16; off <<= 1) {
+    for (int off = 1; off < kWaves; off <<= 1) {
       int o = __shfl_up(s, off, kWave);
       if (lane >= off) s += o;
     }
-    if (lane < kLoopWaves) wsum[lane] = s;
+    if (lane < kWaves) wsum[lane] = s;
   }
   __syncthreads();
   if (wid > 0) v += wsum[wid - 1];
   return v;
 }
 
-__global__ __launch_bounds__(1024) void site_scan_chunks(SiteArgs a) {
-  __shared__ int red[kLoopWaves];
-  const int i = blockIdx.x * 1024 + threadIdx.x;
+__global__ __launch_bounds__(kScanThreads) void site_scan_chunks(SiteArgs a) {
+  __shared__ int red[kScanThreads / kWave];
+  const int i = blockIdx.x * kScanThreads + threadIdx.x;
   int v = i < a.nbuckets ? a.bcount[i] : 0;
   v = wave_sum_i32(v);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
     int s = 0;
-    for (int w = 0; w < kLoopWaves; w++) s += red[w];
+    for (int w = 0; w < kScanThreads / kWave; w++) s += red[w];
     a.chunk[blockIdx.x] = s;
   }
 }
 
 // bstart = exclusive scan of bcount; bcount becomes the scatter cursor
-__global__ __launch_bounds__(1024) void site_scan_apply(SiteArgs a) {
-  __shared__ int wsum[kLoopWaves];
+__global__ __launch_bounds__(kScanThreads) void site_scan_apply(SiteArgs a) {
+  __shared__ int wsum[kScanThreads / kWave];
   __shared__ int prefix;
   int p = 0;
-  for (int b = threadIdx.x; b < (int)blockIdx.x; b += 1024) p += a.chunk[b];
+  for (int b = threadIdx.x; b < (int)blockIdx.x; b += kScanThreads) p += a.chunk[b];
   p = wave_sum_i32(p);
   if (threadIdx.x == 0) prefix = 0;
   __syncthreads();
   if ((threadIdx.x & 63) == 0) atomicAdd(&prefix, p);
   __syncthreads();
-  const int i = blockIdx.x * 1024 + threadIdx.x;
+  const int i = blockIdx.x * kScanThreads + threadIdx.x;
   const int v = i < a.nbuckets ? a.bcount[i] : 0;
-  const int inc = block_inclusive_scan(v, wsum);
+  const int inc = block_inclusive_scan<kScanThreads>(v, wsum);
   const int ex = prefix + inc - v;
   if (i < a.nbuckets) {
     a.bstart[i] = ex;
@@ -768,7 +773,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     }
     {
       const int v = tid < nrng ? rng_end[tid] : 0;
-      const int inc = block_inclusive_scan(v, wsum);
+      const int inc = block_inclusive_scan<kLoopThreads>(v, wsum);
       __syncthreads();
       if (tid < nrng) rng_end[tid] = inc;
     }
@@ -866,7 +871,9 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         if (lcnt == 0) break;
         for (int e = tid; e < lcnt; e += kLoopThreads) list_acc[e] = 0;
         __syncthreads();
-        const int gw = lcnt == 1 ? 16 : lcnt == 2 ? 8 : lcnt <= 4 ? 4 : lcnt <= 8 ? 2 : 1;
+        // waves per candidate: kLoopWaves / lcnt rounded down to a power of two
+        int gw = 1;
+        while (2 * gw * lcnt <= kLoopWaves) gw *= 2;
         const int ngroups = kLoopWaves / gw;
         const int grp = wid / gw, gl = (wid - grp * gw) * kWave + lane;
         for (int e = grp; e < lcnt; e += ngroups) {
@@ -998,6 +1005,7 @@ Plan make_plan(int n, int roi, int nsheds, int groups_cap, int fixed_groups = 0)
   p.Q = choose_q(n, roi, nsheds);
   if (p.Q > p.ncr) p.Q = p.ncr;
   p.nunits = p.ncr * p.Q;
+  // units in a pick's 4*roi window, times workgroups per such unit
   const int touched = ((4 * roi) / p.cs + 2) * p.Q;
   int G = fixed_groups > 0 ? fixed_groups : touched;
   G = G < p.nunits ? G : p.nunits;
@@ -1007,7 +1015,7 @@ Plan make_plan(int n, int roi, int nsheds, int groups_cap, int fixed_groups = 0)
   p.upw = (p.nunits + p.G - 1) / p.G;
   p.kpu = (p.ncr + p.Q - 1) / p.Q;
   p.nbuckets = p.G * p.upw * p.kpu;
-  p.nchunks = (p.nbuckets + 1023) / 1024;
+  p.nchunks = (p.nbuckets + kScanThreads - 1) / kScanThreads;
   const int nwpr = (2 * roi + 1 + 63) / 64;
   const size_t delta_bytes = sizeof(uint64_t) * (size_t)(2 * roi + 1) * (nwpr + 1);
   p.nbwords = (size_t)kMaxBatch * (2 * roi + 1) * (nwpr + 1);
@@ -1075,7 +1083,7 @@ size_t scratch_layout(const Plan &p, int nsheds, uint8_t *base, Scratch *out) {
   const size_t nb_max = ((size_t)p.nunits + kMaxGroups) * p.kpu;
   s.bcount = (int32_t *)take(4 * nb_max);
   s.bstart = (int32_t *)take(4 * (nb_max + 1));
-  s.chunk = (int32_t *)take(4 * ((nb_max + 1023) / 1024));
+  s.chunk = (int32_t *)take(4 * ((nb_max + kScanThreads - 1) / kScanThreads));
   s.sid = (int32_t *)take(4 * (size_t)nsheds);
   s.spos = (uint32_t *)take(4 * (size_t)nsheds);
   s.sT = (int32_t *)take(4 * (size_t)nsheds);
@@ -1166,12 +1174,12 @@ siting_status site_make_args(const Geometry &g, const int32_t *obs,
 
 siting_status site_launch_prep(const SiteArgs &a, hipStream_t s) {
   const int tb = 256, nbk = (a.nsheds + tb - 1) / tb;
-  const int nchunks = (a.nbuckets + 1023) / 1024;
+  const int nchunks = (a.nbuckets + kScanThreads - 1) / kScanThreads;
   hipLaunchKernelGGL(site_zero_counts, dim3((a.nbuckets + 255) / 256), dim3(256), 0, s,
                      a.bcount, a.nbuckets);
   hipLaunchKernelGGL(site_bucket_count, dim3(nbk), dim3(tb), 0, s, a);
-  hipLaunchKernelGGL(site_scan_chunks, dim3(nchunks), dim3(1024), 0, s, a);
-  hipLaunchKernelGGL(site_scan_apply, dim3(nchunks), dim3(1024), 0, s, a);
+  hipLaunchKernelGGL(site_scan_chunks, dim3(nchunks), dim3(kScanThreads), 0, s, a);
+  hipLaunchKernelGGL(site_scan_apply, dim3(nchunks), dim3(kScanThreads), 0, s, a);
   hipLaunchKernelGGL(site_bucket_scatter, dim3(nbk), dim3(tb), 0, s, a);
   hipLaunchKernelGGL(site_first_pass, dim3((a.nsheds + 3) / 4), dim3(256), 0, s, a);
   SITING_HIP(hipGetLastError());
@@ -1191,9 +1199,9 @@ siting_status launch_site(const Geometry &g, const int32_t *obs,
   SITING_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
                                    device));
   // The grid barrier needs every workgroup of the loop resident at once; a
-  // grid of at most half the CUs (one 1024-thread group per CU) is, since any
-  // other kernel holding CUs when it starts does not wait on it and retires.
-  // A plain launch is used: a cooperative launch reserves a device-wide
+  // grid of at most half the CUs (one group per CU) is, since any other
+  // kernel holding CUs when it starts does not wait on it and retires.  A
+  // plain launch is used: a cooperative launch reserves a device-wide
   // resource that a second process on the same GPU cannot get.
   int cap = cus / 2 > 0 ? cus / 2 : 1;
   if (groups_cap > 0 && groups_cap < cap) cap = groups_cap;
