@@ -180,9 +180,11 @@ siting_status siting_job_derived(siting_job *job, siting_derived *d);
  * the per-job VIX, FINDMAX, VIEWSHED and SITE-preparation kernels go to a
  * small pool of streams so they overlap, and every job's greedy SITE loop
  * runs in a single persistent work-queue launch whose slots of workgroups
- * take jobs, largest first, as they free up.  The launch sequence can be
- * captured once into a HIP graph and replayed.  Results are bit-identical
- * to running each job alone. */
+ * take jobs, largest first, as they free up.  After the first synced run the
+ * slots are re-planned once from the measured loop time of every job: the
+ * longest jobs get a few larger slots.  The prep sequence is captured into a
+ * HIP graph and replayed.  Results are bit-identical to running each job
+ * alone, whatever slot runs it. */
 
 typedef struct siting_batch siting_batch;
 
@@ -191,6 +193,9 @@ typedef struct {
   int slot;      /* workgroups per SITE slot (0: tuned default) */
   int nslots;    /* SITE slots (0: CUs / slot) */
   int use_graph; /* 1: capture the run into a HIP graph on first use */
+  int fixed_slots; /* 1: keep equal slots; 0: after the first synced run,
+                      re-plan the slots (a few larger ones for the longest
+                      jobs) from each job's measured SITE loop time */
 } siting_batch_opts;
 
 siting_status siting_batch_create(int device, int njobs, const siting_params *params,

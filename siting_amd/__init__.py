@@ -329,7 +329,8 @@ class Job:
 
 class BatchOpts(ctypes.Structure):
     _fields_ = [("streams", ctypes.c_int), ("slot", ctypes.c_int),
-                ("nslots", ctypes.c_int), ("use_graph", ctypes.c_int)]
+                ("nslots", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("fixed_slots", ctypes.c_int)]
 
 
 class _JobView(Job):
@@ -349,10 +350,12 @@ class _JobView(Job):
 class Batch:
     """Many independent jobs run as one unit (see siting.h, batched jobs)."""
 
-    def __init__(self, params_list, device=0, streams=0, slot=0, nslots=0, use_graph=False):
+    def __init__(self, params_list, device=0, streams=0, slot=0, nslots=0, use_graph=False,
+                 fixed_slots=False):
         arr = (Params * len(params_list))(*params_list)
         self.params = list(params_list)
-        opts = BatchOpts(streams, slot, nslots, 1 if use_graph else 0)
+        opts = BatchOpts(streams, slot, nslots, 1 if use_graph else 0,
+                         1 if fixed_slots else 0)
         h = ctypes.c_void_p()
         _check(lib().siting_batch_create(device, len(params_list), ctypes.cast(arr, _VP),
                                          ctypes.byref(opts), ctypes.byref(h)))

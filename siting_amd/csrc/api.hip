@@ -1,5 +1,6 @@
 // api.hip -- the C ABI (include/siting.h): parameter derivation, device
 // resident jobs, and the reference-shaped stage functions on host arrays.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -587,23 +588,81 @@ siting_status siting_site_it(int nrows, int roi, int intervis, int nsheds,
 // batches
 // ---------------------------------------------------------------------------
 struct siting_batch {
-  int device = 0;
+  int device = 0, cus = 0;
   std::vector<siting_job *> jobs;
   std::vector<hipStream_t> streams;
   std::vector<hipEvent_t> joins;
   hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
-  hipEvent_t ts = nullptr;         // before the SITE work-queue launch
-  std::vector<int> order;          // jobs by descending SITE work
-  SiteArgs *args_dev = nullptr;    // [njobs] in queue order
-  unsigned int *ctl = nullptr;     // work-queue counters
+  hipEvent_t ts = nullptr;             // before the SITE work-queue launch
+  std::vector<int> order;              // prep order: descending SITE work
+  std::vector<int> cap;                // per job: workgroups of its slot class
+  SiteArgs *args_dev = nullptr;        // [njobs], by job index
+  int *plan_dev = nullptr;             // slot and queue plan
+  unsigned int *ctl = nullptr;         // work-queue state
+  unsigned long long *jobt = nullptr;  // [2 * njobs] loop start/end per job
   size_t lds = 0;
-  int slot = 16, nslots = 1;
-  bool use_graph = false;
+  int slot = 16, nslots = 1, nclasses = 1, total_groups = 16;
+  bool use_graph = false, replan = true, replanned = false;
+  int runs = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
 
 namespace {
+
+// Host form of the work-queue plan (layout in common.hpp).
+struct QueuePlan {
+  std::vector<int> start, size, cls, off, order;
+  int total() const {
+    int t = 0;
+    for (int v : size) t += v;
+    return t;
+  }
+};
+
+// Uploads a plan and re-plans every job for its class's slot size.
+siting_status batch_set_plan(siting_batch *b, const QueuePlan &p,
+                             const std::vector<int> &cap) {
+  const int njobs = (int)b->jobs.size();
+  std::vector<SiteArgs> args((size_t)njobs);
+  size_t lds_max = 0;
+  for (int i = 0; i < njobs; i++) {
+    siting_job *j = b->jobs[(size_t)i];
+    size_t lds = 0;
+    siting_status st = site_make_args(j->g, j->obs, j->sheds, j->site, cap[(size_t)i],
+                                      &args[(size_t)i], &lds);
+    if (st != SITING_OK) return st;
+    lds_max = lds > lds_max ? lds : lds_max;
+  }
+  std::vector<int> packed;
+  packed.push_back((int)p.start.size());
+  packed.push_back((int)p.off.size() - 1);
+  for (const auto *v : {&p.start, &p.size, &p.cls, &p.off, &p.order})
+    packed.insert(packed.end(), v->begin(), v->end());
+  SITING_HIP(hipMemcpy(b->args_dev, args.data(), sizeof(SiteArgs) * (size_t)njobs,
+                       hipMemcpyHostToDevice));
+  SITING_HIP(hipMemcpy(b->plan_dev, packed.data(), sizeof(int) * packed.size(),
+                       hipMemcpyHostToDevice));
+  b->cap = cap;
+  b->lds = lds_max;
+  b->nslots = (int)p.start.size();
+  b->nclasses = (int)p.off.size() - 1;
+  b->total_groups = p.total();
+  return SITING_OK;
+}
+
+// nslots slots of `slot` workgroups, one queue in prep order.
+QueuePlan uniform_plan(const siting_batch *b) {
+  QueuePlan p;
+  for (int s = 0; s < b->nslots; s++) {
+    p.start.push_back(s * b->slot);
+    p.size.push_back(b->slot);
+    p.cls.push_back(0);
+  }
+  p.off = {0, (int)b->order.size()};
+  p.order = b->order;
+  return p;
+}
 
 // Every job's VIX, FINDMAX, VIEWSHED and SITE preparation, spread over the
 // stream pool and joined back into streams[0] (the part a HIP graph holds).
@@ -613,7 +672,8 @@ siting_status batch_enqueue_prep(siting_batch *b) {
   for (size_t k = 1; k < b->streams.size(); k++)
     SITING_HIP(hipStreamWaitEvent(b->streams[k], b->fork, 0));
   for (size_t q = 0; q < b->order.size(); q++) {
-    siting_job *j = b->jobs[(size_t)b->order[q]];
+    const int i = b->order[q];
+    siting_job *j = b->jobs[(size_t)i];
     hipStream_t s = b->streams[q % b->streams.size()];
     siting_status st;
     if ((st = launch_vix(j->g, j->elevs, j->vix, VixTables{j->recip, j->jump}, s)) != SITING_OK)
@@ -622,8 +682,8 @@ siting_status batch_enqueue_prep(siting_batch *b) {
     if ((st = launch_viewshed(j->g, j->elevs, j->obs, j->sheds, j->rdiv, s)) != SITING_OK)
       return st;
     SiteArgs a;
-    if ((st = site_make_args(j->g, j->obs, j->sheds, j->site, b->slot, &a, nullptr)) !=
-        SITING_OK)
+    if ((st = site_make_args(j->g, j->obs, j->sheds, j->site, b->cap[(size_t)i], &a,
+                             nullptr)) != SITING_OK)
       return st;
     if ((st = site_launch_prep(a, s)) != SITING_OK) return st;
   }
@@ -639,11 +699,131 @@ siting_status batch_enqueue_prep(siting_batch *b) {
 siting_status batch_enqueue_site(siting_batch *b) {
   hipStream_t s0 = b->streams[0];
   SITING_HIP(hipEventRecord(b->ts, s0));
-  siting_status st = site_launch_queue(b->args_dev, (int)b->jobs.size(), b->slot,
-                                       b->nslots, b->lds, b->ctl, s0);
+  siting_status st = site_launch_queue(b->args_dev, b->plan_dev, b->nslots, b->nclasses,
+                                       b->total_groups, b->lds, b->ctl, b->jobt, s0);
   if (st != SITING_OK) return st;
   SITING_HIP(hipEventRecord(b->t1, s0));
   return SITING_OK;
+}
+
+// Captures the prep sequence into a HIP graph (executes nothing).
+siting_status batch_capture(siting_batch *b) {
+  hipStream_t s0 = b->streams[0];
+  SITING_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
+  siting_status st = batch_enqueue_prep(b);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s0, &g);
+  if (st != SITING_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+  b->graph = g;
+  SITING_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return SITING_OK;
+}
+
+void batch_drop_graph(siting_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+// Slot classes from the measured loop times of the last run.  A job that
+// took t ms on G workgroups is modelled to take t*sqrt(G/G') on G' (the
+// loop's rounds are latency-bound, so more groups help less than linearly).
+// Jobs, longest first, go to whichever slot would finish them first; the
+// mix of big slots (32, 48 or 64 groups, one to four of them) and slots of
+// the base size with the smallest finish time is kept if it beats the
+// uniform slots by 10 %.  Each class then queues its jobs longest first.
+siting_status batch_replan(siting_batch *b) {
+  const int njobs = (int)b->jobs.size();
+  std::vector<unsigned long long> t((size_t)2 * njobs);
+  SITING_HIP(hipMemcpy(t.data(), b->jobt, sizeof(unsigned long long) * t.size(),
+                       hipMemcpyDeviceToHost));
+  std::vector<double> ms((size_t)njobs), g0((size_t)njobs);
+  for (int i = 0; i < njobs; i++) {
+    const unsigned long long a = t[(size_t)2 * i], e = t[(size_t)2 * i + 1];
+    ms[(size_t)i] = e > a ? (double)(e - a) * 1e-5 : 0.0;  // 100 MHz clock
+    g0[(size_t)i] = site_loop_groups(b->jobs[(size_t)i]->g, b->cus, b->cap[(size_t)i]);
+  }
+  auto time_on = [&](int i, int groups) {
+    const double g = site_loop_groups(b->jobs[(size_t)i]->g, b->cus, groups);
+    return ms[(size_t)i] * std::sqrt(g0[(size_t)i] / g);
+  };
+  std::vector<int> by_time((size_t)njobs);
+  for (int i = 0; i < njobs; i++) by_time[(size_t)i] = i;
+  std::stable_sort(by_time.begin(), by_time.end(),
+                   [&](int x, int y) { return ms[(size_t)x] > ms[(size_t)y]; });
+  const int total = b->total_groups;
+  // list schedule over nb slots of size sb and slots of the base size;
+  // returns the makespan, fills each job's class (0 big, 1 base)
+  auto schedule = [&](int nb, int sb, std::vector<int> *cls) {
+    std::vector<double> fin;
+    std::vector<int> size;
+    for (int k = 0; k < nb; k++) size.push_back(sb);
+    for (int k = 0; k < (total - nb * sb) / b->slot; k++) size.push_back(b->slot);
+    fin.assign(size.size(), 0.0);
+    if (cls) cls->assign((size_t)njobs, 1);
+    for (int i : by_time) {
+      size_t best = 0;
+      double bf = 1e300;
+      for (size_t k = 0; k < size.size(); k++) {
+        const double f = fin[k] + time_on(i, size[k]);
+        if (f < bf) {
+          bf = f;
+          best = k;
+        }
+      }
+      fin[best] = bf;
+      if (cls) (*cls)[(size_t)i] = size[best] == b->slot && (int)best >= nb ? 1 : 0;
+    }
+    double m = 0;
+    for (double f : fin) m = f > m ? f : m;
+    return m;
+  };
+  const double m0 = schedule(0, b->slot, nullptr);
+  int best_nb = 0, best_sb = 0;
+  double best_m = m0;
+  for (int sb : {32, 48, 64}) {
+    if (sb <= b->slot) continue;
+    for (int nb = 1; nb <= 4; nb++) {
+      if (total - nb * sb < b->slot) break;
+      const double m = schedule(nb, sb, nullptr);
+      if (m < best_m) {
+        best_m = m;
+        best_nb = nb;
+        best_sb = sb;
+      }
+    }
+  }
+  if (best_nb == 0 || best_m > 0.9 * m0) return SITING_OK;
+  std::vector<int> cls;
+  schedule(best_nb, best_sb, &cls);
+  QueuePlan p;
+  const int nsmall = (total - best_nb * best_sb) / b->slot;
+  for (int k = 0; k < best_nb; k++) {
+    p.start.push_back(k * best_sb);
+    p.size.push_back(best_sb);
+    p.cls.push_back(0);
+  }
+  for (int k = 0; k < nsmall; k++) {
+    p.start.push_back(best_nb * best_sb + k * b->slot);
+    p.size.push_back(b->slot);
+    p.cls.push_back(1);
+  }
+  std::vector<int> cap((size_t)njobs);
+  std::vector<int> q0, q1;
+  for (int i : by_time) {
+    (cls[(size_t)i] == 0 ? q0 : q1).push_back(i);
+    cap[(size_t)i] = cls[(size_t)i] == 0 ? best_sb : b->slot;
+  }
+  p.off = {0, (int)q0.size(), (int)(q0.size() + q1.size())};
+  p.order = q0;
+  p.order.insert(p.order.end(), q1.begin(), q1.end());
+  batch_drop_graph(b);
+  return batch_set_plan(b, p, cap);
 }
 
 }  // namespace
@@ -655,8 +835,7 @@ void siting_batch_destroy(siting_batch *b) {
   (void)hipSetDevice(b->device);
   for (hipStream_t s : b->streams)
     if (s) (void)hipStreamSynchronize(s);
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
+  batch_drop_graph(b);
   for (hipEvent_t e : b->joins)
     if (e) (void)hipEventDestroy(e);
   if (b->fork) (void)hipEventDestroy(b->fork);
@@ -666,7 +845,9 @@ void siting_batch_destroy(siting_batch *b) {
   for (hipStream_t s : b->streams)
     if (s) (void)hipStreamDestroy(s);
   if (b->args_dev) (void)hipFree(b->args_dev);
+  if (b->plan_dev) (void)hipFree(b->plan_dev);
   if (b->ctl) (void)hipFree(b->ctl);
+  if (b->jobt) (void)hipFree(b->jobt);
   for (siting_job *j : b->jobs) siting_job_destroy(j);
   delete b;
 }
@@ -685,10 +866,11 @@ siting_status siting_batch_create(int device, int njobs, const siting_params *pa
     siting_job *j = nullptr;
     siting_status st = siting_job_create(device, &params[i], &j);
     if (st != SITING_OK) return fail(st);
+    j->site_state = true;  // stats read the SITE state
     b->jobs.push_back(j);
   }
-  int cus = 0;
-  SITING_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+  SITING_HIP(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device));
+  const int cus = b->cus;
   const int nstreams = opts && opts->streams > 0 ? opts->streams : 4;
   // 16 workgroups per slot measured best on the uneven 64-job batch
   // (8: 640 ms, 16: 499, 32: 525, 64: 692)
@@ -699,6 +881,7 @@ siting_status siting_batch_create(int device, int njobs, const siting_params *pa
   if (b->nslots < 1) b->nslots = 1;
   if (b->nslots > njobs) b->nslots = njobs;
   b->use_graph = opts && opts->use_graph;
+  b->replan = !(opts && opts->fixed_slots);
   hipError_t e;
   for (int k = 0; k < nstreams; k++) {
     hipStream_t s = nullptr;
@@ -714,7 +897,8 @@ siting_status siting_batch_create(int device, int njobs, const siting_params *pa
       (e = hipEventCreate(&b->t0)) != hipSuccess || (e = hipEventCreate(&b->t1)) != hipSuccess ||
       (e = hipEventCreate(&b->ts)) != hipSuccess)
     return fail(hip_fail(e, "hipEventCreate", __FILE__, __LINE__));
-  // queue order: largest SITE work first (candidates x shed words)
+  // prep order and first queue order: largest SITE work first (candidates x
+  // shed words)
   b->order.resize((size_t)njobs);
   for (int i = 0; i < njobs; i++) b->order[(size_t)i] = i;
   std::vector<double> work((size_t)njobs);
@@ -723,22 +907,17 @@ siting_status siting_batch_create(int device, int njobs, const siting_params *pa
                       (double)b->jobs[(size_t)i]->g.d.shed_words;
   std::stable_sort(b->order.begin(), b->order.end(),
                    [&](int x, int y) { return work[(size_t)x] > work[(size_t)y]; });
-  std::vector<SiteArgs> args((size_t)njobs);
-  for (int q = 0; q < njobs; q++) {
-    siting_job *j = b->jobs[(size_t)b->order[(size_t)q]];
-    size_t lds = 0;
-    siting_status st = site_make_args(j->g, j->obs, j->sheds, j->site, b->slot,
-                                      &args[(size_t)q], &lds);
-    if (st != SITING_OK) return fail(st);
-    b->lds = lds > b->lds ? lds : b->lds;
-    j->site_state = true;  // stats read the SITE state
-  }
+  // buffers sized for any plan the re-planning may choose: at most
+  // nslots slots (big slots only replace base ones) and two classes
+  const int max_slots = b->nslots;
   if ((e = hipMalloc(&b->args_dev, sizeof(SiteArgs) * (size_t)njobs)) != hipSuccess ||
-      (e = hipMalloc(&b->ctl, 4 * site_queue_ctl_words(b->nslots))) != hipSuccess)
+      (e = hipMalloc(&b->plan_dev, sizeof(int) * site_queue_plan_ints(max_slots, 2, njobs))) !=
+          hipSuccess ||
+      (e = hipMalloc(&b->ctl, 4 * site_queue_ctl_words(max_slots, 2))) != hipSuccess ||
+      (e = hipMalloc(&b->jobt, sizeof(unsigned long long) * 2 * (size_t)njobs)) != hipSuccess)
     return fail(hip_fail(e, "hipMalloc(batch)", __FILE__, __LINE__));
-  if ((e = hipMemcpy(b->args_dev, args.data(), sizeof(SiteArgs) * (size_t)njobs,
-                     hipMemcpyHostToDevice)) != hipSuccess)
-    return fail(hip_fail(e, "hipMemcpy(batch args)", __FILE__, __LINE__));
+  siting_status st = batch_set_plan(b, uniform_plan(b), std::vector<int>((size_t)njobs, b->slot));
+  if (st != SITING_OK) return fail(st);
   *out = b;
   return SITING_OK;
 }
@@ -751,22 +930,14 @@ siting_status siting_batch_upload_dem(siting_batch *b, int job, const uint16_t *
 siting_status siting_batch_run(siting_batch *b) {
   if (!b) return SITING_EINVAL;
   SITING_HIP(hipSetDevice(b->device));
+  b->runs++;
   hipStream_t s0 = b->streams[0];
+  if (b->use_graph && !b->exec) {
+    siting_status st = batch_capture(b);
+    if (st != SITING_OK) return st;
+  }
   SITING_HIP(hipEventRecord(b->t0, s0));
   if (b->use_graph) {
-    if (!b->exec) {
-      SITING_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
-      siting_status st = batch_enqueue_prep(b);
-      hipGraph_t g = nullptr;
-      hipError_t e = hipStreamEndCapture(s0, &g);
-      if (st != SITING_OK) {
-        if (g) (void)hipGraphDestroy(g);
-        return st;
-      }
-      if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
-      b->graph = g;
-      SITING_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-    }
     SITING_HIP(hipGraphLaunch(b->exec, s0));
   } else {
     siting_status st = batch_enqueue_prep(b);
@@ -779,6 +950,12 @@ siting_status siting_batch_sync(siting_batch *b) {
   if (!b) return SITING_EINVAL;
   SITING_HIP(hipSetDevice(b->device));
   for (hipStream_t s : b->streams) SITING_HIP(hipStreamSynchronize(s));
+  if (b->replan && !b->replanned && b->runs > 0) {
+    b->replanned = true;
+    siting_status st = batch_replan(b);
+    if (st != SITING_OK) return st;
+    if (b->use_graph && !b->exec) return batch_capture(b);
+  }
   return SITING_OK;
 }
 

@@ -184,14 +184,24 @@ int site_loop_groups(const Geometry &g, int device_cus, int groups_cap);
 // Batched SITE: site_make_args plans one job for a work-queue slot of `slot`
 // workgroups (no launch); site_launch_prep enqueues its preparation kernels;
 // site_launch_queue runs every job's loop in one persistent launch of
-// nslots * slot workgroups.  ctl holds site_queue_ctl_words(nslots) words.
+// total_groups workgroups split into slots.
+//   plan (device ints, site_queue_plan_ints of them): nslots, nclasses,
+//     start[nslots] (first workgroup, increasing), size[nslots],
+//     cls[nslots], off[nclasses + 1], order[njobs]: class c's queue is
+//     order[off[c] .. off[c+1]), indices into the SiteArgs array, and every
+//     job of class c is planned for at most the size of class c's slots.
+//   ctl: site_queue_ctl_words(nslots, nclasses) words of queue state.
+//   jobt: 2 words per job, its loop's start and end on the device clock
+//     (wall_clock64, 100 MHz).
 siting_status site_make_args(const Geometry &g, const int32_t *obs,
                              const uint64_t *sheds, const SiteBuffers &b, int slot,
                              SiteArgs *out, size_t *lds);
 siting_status site_launch_prep(const SiteArgs &a, hipStream_t s);
-size_t site_queue_ctl_words(int nslots);
-siting_status site_launch_queue(const SiteArgs *jobs_dev, int njobs, int slot,
-                                int nslots, size_t lds, unsigned int *ctl,
+size_t site_queue_ctl_words(int nslots, int nclasses);
+size_t site_queue_plan_ints(int nslots, int nclasses, int njobs);
+siting_status site_launch_queue(const SiteArgs *jobs_dev, const int *plan_dev,
+                                int nslots, int nclasses, int total_groups, size_t lds,
+                                unsigned int *ctl, unsigned long long *jobt,
                                 hipStream_t s);
 
 // host tables for VIX

@@ -433,7 +433,6 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   __shared__ uint64_t bw_key[kMaxBatch];
   __shared__ uint32_t bw_pos[kMaxBatch];
   __shared__ int bw_n;
-  __shared__ int bw_live;
   __shared__ int bw_head;
 
   const int tid = threadIdx.x;
@@ -931,26 +930,44 @@ __global__ __launch_bounds__(kLoopThreads) void site_loop(SiteArgs a) {
 // idle), then takes another, until the queue is empty.  The job index is
 // double-buffered by the slot's turn so a late reader never sees the next
 // turn's value.
+// Slots of workgroups pop jobs from their class's queue (plan layout in
+// site_queue_plan_ints) and run each job's loop with the slot's first G
+// workgroups; jobt[j] receives the loop's start and end (device clock).
 __global__ __launch_bounds__(kLoopThreads) void site_loop_queue(
-    const SiteArgs *jobs, int njobs, int slot, unsigned int *queue,
-    unsigned int *slot_job, unsigned int *slot_bar) {
+    const SiteArgs *jobs, const int *plan, unsigned int *ctl,
+    unsigned long long *jobt) {
   __shared__ int job;
-  const int sl = blockIdx.x / slot, g = blockIdx.x - sl * slot;
+  const int nslots = plan[0], ncls = plan[1];
+  const int *start = plan + 2, *size = start + nslots, *cls = size + nslots;
+  const int *off = cls + nslots, *order = off + ncls + 1;
+  int sl = 0;  // slots are listed by first workgroup
+  while (sl + 1 < nslots && (int)blockIdx.x >= start[sl + 1]) sl++;
+  const int g = blockIdx.x - start[sl], slot = size[sl], c = cls[sl];
+  unsigned int *queue = ctl + c;
+  unsigned int *slot_job = ctl + ncls + 2 * sl;
+  unsigned int *slot_bar = ctl + ncls + 2 * nslots + sl;
+  const int qlen = off[c + 1] - off[c];
   unsigned int target = 0;
   for (int turn = 0;; turn++) {
-    unsigned int *cell = &slot_job[2 * sl + (turn & 1)];
+    unsigned int *cell = &slot_job[turn & 1];
     if (g == 0 && threadIdx.x == 0)
       __hip_atomic_store(cell, atomicAdd(queue, 1u), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
-    grid_sync(&slot_bar[sl], target, (unsigned)slot);
+    grid_sync(slot_bar, target, (unsigned)slot);
     if (threadIdx.x == 0)
       job = (int)__hip_atomic_load(cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
-    const int j = job;
-    if (j >= njobs) break;
+    const int idx = job;
+    if (idx >= qlen) break;
+    const int j = order[off[c] + idx];
     const SiteArgs &a = jobs[j];
+    const unsigned long long t0 = wall_clock64();
     if (g < a.G) site_loop_body(a, g, a.G);
     __syncthreads();
+    if (g == 0 && threadIdx.x == 0) {
+      jobt[2 * j] = t0;
+      jobt[2 * j + 1] = wall_clock64();
+    }
   }
 }
 
@@ -1221,17 +1238,23 @@ __global__ void site_zero_ctl(unsigned int *ctl, int words) {
   if (i < words) ctl[i] = 0;
 }
 
-size_t site_queue_ctl_words(int nslots) { return 1 + 3 * (size_t)nslots; }
+size_t site_queue_ctl_words(int nslots, int nclasses) {
+  return (size_t)nclasses + 3 * (size_t)nslots;
+}
 
-siting_status site_launch_queue(const SiteArgs *jobs_dev, int njobs, int slot,
-                                int nslots, size_t lds, unsigned int *ctl,
+size_t site_queue_plan_ints(int nslots, int nclasses, int njobs) {
+  return 2 + 3 * (size_t)nslots + (size_t)nclasses + 1 + (size_t)njobs;
+}
+
+siting_status site_launch_queue(const SiteArgs *jobs_dev, const int *plan_dev,
+                                int nslots, int nclasses, int total_groups, size_t lds,
+                                unsigned int *ctl, unsigned long long *jobt,
                                 hipStream_t s) {
-  const int words = (int)site_queue_ctl_words(nslots);
+  const int words = (int)site_queue_ctl_words(nslots, nclasses);
   hipLaunchKernelGGL(site_zero_ctl, dim3((words + 255) / 256), dim3(256), 0, s, ctl,
                      words);
-  unsigned int *queue = ctl, *slot_job = ctl + 1, *slot_bar = ctl + 1 + 2 * nslots;
-  hipLaunchKernelGGL(site_loop_queue, dim3(nslots * slot), dim3(kLoopThreads), lds, s,
-                     jobs_dev, njobs, slot, queue, slot_job, slot_bar);
+  hipLaunchKernelGGL(site_loop_queue, dim3(total_groups), dim3(kLoopThreads), lds, s,
+                     jobs_dev, plan_dev, ctl, jobt);
   SITING_HIP(hipGetLastError());
   return SITING_OK;
 }

@@ -29,14 +29,17 @@ def refs(sa, orc):
     return out
 
 
-@pytest.mark.parametrize("use_graph,slot", [(False, 16), (True, 16), (True, 4)])
-def test_batch_matches_oracle(gpu, refs, use_graph, slot):
-    print(f"[case] batch graph={use_graph} slot={slot}", flush=True)
+@pytest.mark.parametrize("use_graph,slot,fixed", [(False, 16, True), (True, 16, False),
+                                                 (True, 4, False), (False, 8, False)])
+def test_batch_matches_oracle(gpu, refs, use_graph, slot, fixed):
+    print(f"[case] batch graph={use_graph} slot={slot} fixed={fixed}", flush=True)
     params = [gpu.make_params(*c[:7]) for c in JOBS]
-    with gpu.Batch(params, use_graph=use_graph, slot=slot) as b:
+    with gpu.Batch(params, use_graph=use_graph, slot=slot, fixed_slots=fixed) as b:
         for i, (dem, _) in enumerate(refs):
             b.upload_dem(i, dem)
-        for rep in range(2):  # replay: state is rebuilt each run
+        # replays: state is rebuilt each run, and after the first the slots
+        # may be re-planned from measured times
+        for rep in range(3):
             b.run()
             b.sync()
             for i, (dem, ref) in enumerate(refs):
