@@ -167,15 +167,20 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
     // midpoint, and never on one, while the double product below is within
     // 2^-52 of it, so rounding it to float gives the correctly rounded fp32
     // quotient, bit for bit.
-    auto step = [&](int a, int pel, int qx, int qy) {
-      const float sl = (float)((double)(pel - oalt) * rd[a]);
-      if (horizon < sl) horizon = sl;
+    // Lanes past their ray's last step compute too, and discard the result:
+    // the step stays branch-free up to the atomic, so the wait for the
+    // reciprocals happens once per group.
+    auto step = [&](bool take, int a, double rda, int pel, int qx, int qy) {
+      const float sl = (float)((double)(pel - oalt) * rda);
+      horizon = take && horizon < sl ? sl : horizon;
       const float rise = horizon * (float)a;
       const float halt = foalt + rise;  // siting.cpp:342
-      if ((float)(pel + tht) >= halt) {
+      if (take && (float)(pel + tht) >= halt) {
         const int row = qx + roi, col = qy + roi;
         if (kMode == kSplit) {
-          const int w = along_y ? row * ps + (col >> 6) : nr * ps + col * ps + (row >> 6);
+          // (24-bit products: rows, columns and strides are below 2^15)
+          const int w = along_y ? (int)__umul24(row, ps) + (col >> 6)
+                                : nr * ps + (int)__umul24(col, ps) + (row >> 6);
           const int bit = along_y ? (col & 63) : (row & 63);
           atomicOr((unsigned long long *)&bits[w], 1ull << (63 - bit));
         } else {
@@ -186,17 +191,29 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
     };
     // four steps at a time, their DEM loads issued before any is used; the
     // wavefront runs to its longest ray, each lane stepping up to its own
+    // The four reciprocals are read first (scalar loads share their wait
+    // counter with the LDS atomics, so a read just before its use would also
+    // wait for every atomic in flight; the table has three spare entries
+    // past roi for the last group).  DEM offsets are 32-bit with 24-bit
+    // products (n < 2^15).
     const int wlast = __builtin_amdgcn_readfirstlane(wave_max_i32(last));
     for (int a = 1; a <= wlast; a += 4) {
+      double rda[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) rda[u] = rd[a + u];
       int qx[4], qy[4], pel[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         cell(a + u, qx[u], qy[u]);
-        pel[u] = a + u <= last ? (int)elevs[(ox + qx[u]) * n + (oy + qy[u])] : 0;
+        const uint32_t idx = __umul24((uint32_t)(ox + qx[u]), (uint32_t)n) + (uint32_t)(oy + qy[u]);
+        pel[u] = a + u <= last ? (int)elevs[idx] : 0;
       }
+      // (no instruction: makes the four reciprocals ready here, after the
+      // DEM loads are issued and before the steps' conditional blocks, so
+      // that none of those waits again)
+      asm volatile("" ::"s"(rda[0]), "s"(rda[1]), "s"(rda[2]), "s"(rda[3]));
 #pragma unroll
-      for (int u = 0; u < 4; u++)
-        if (a + u <= last) step(a + u, pel[u], qx[u], qy[u]);
+      for (int u = 0; u < 4; u++) step(a + u <= last, a + u, rda[u], pel[u], qx[u], qy[u]);
     }
   }
 
