@@ -24,6 +24,11 @@ namespace {
 
 constexpr int kTile = 16;           // tile edge (points)
 constexpr int kThreads = kTile * kTile;
+// The tile's DEM window with a halo of kHalo cells stays in LDS: samples at
+// steps up to kHalo from an observer of the tile (the minor offset is never
+// larger than the step) and the observers' own elevations are read from it.
+template <int kHalo>
+constexpr int win_edge() { return kTile + 2 * kHalo; }
 
 // siting.cpp:157,160, evaluated in float in the reference's order:
 // ((float)(2roi) + 0.99999f) * (float)r / 2^31, truncated.
@@ -33,7 +38,7 @@ __device__ __forceinline__ int target_offset(float span, uint32_t r) {
   return (int)q;
 }
 
-template <int kGroup>
+template <int kGroup, int kHalo>
 __global__ __launch_bounds__(kThreads) void vix_kernel(
     const uint16_t *__restrict__ elevs, uint8_t *__restrict__ vix,
     const float *__restrict__ recip, const uint32_t *__restrict__ jump,
@@ -42,6 +47,8 @@ __global__ __launch_bounds__(kThreads) void vix_kernel(
   extern __shared__ uint32_t lds[];
   uint32_t *jl = lds;                      // 4*ntests: A1 C1 A2 C2 per target
   int *count = (int *)(lds + 4 * ntests);  // kThreads
+  constexpr int kWin = win_edge<kHalo>();
+  uint16_t *win = (uint16_t *)(count + kThreads);  // kWin x kWin
 
   const int b = blockIdx.x;
   const int tile = (b & 7) * tiles_per_xcd + (b >> 3);
@@ -51,6 +58,13 @@ __global__ __launch_bounds__(kThreads) void vix_kernel(
 
   for (int i = threadIdx.x; i < 4 * ntests; i += kThreads) jl[i] = jump[i];
   count[threadIdx.x] = 0;
+  const int wx0 = tx0 - kHalo, wy0 = ty0 - kHalo;
+  for (int i = threadIdx.x; i < kWin * kWin; i += kThreads) {
+    const int wx = wx0 + i / kWin, wy = wy0 + i % kWin;
+    // cells outside the terrain are never read (samples lie between an
+    // observer and a target, both inside)
+    win[i] = wx >= 0 && wx < n && wy >= 0 && wy < n ? elevs[wx * n + wy] : 0;
+  }
   __syncthreads();
 
   const float span = (float)(2 * roi) + 0.99999f;
@@ -96,7 +110,7 @@ __global__ __launch_bounds__(kThreads) void vix_kernel(
           // target is visible iff no sample is above it: the samples can be
           // read in any order.  kGroup at a time, the first group in flight
           // together with the observer and target elevations.
-          const int oz_raw = (int)elevs[p];
+          const int oz_raw = (int)win[(px - wx0) * kWin + (py - wy0)];
           const int tz_raw = (int)elevs[tx * n + ty];
           float zz[kGroup];
           auto load_group = [&](int s0) {
@@ -107,7 +121,9 @@ __global__ __launch_bounds__(kThreads) void vix_kernel(
               const int off = round_half_away(slope * (float)id);
               const int sx = alx ? px + id : px + off;
               const int sy = alx ? py + off : py + id;
-              zz[u] = sv < len ? (float)elevs[sx * n + sy] : 0.0f;
+              zz[u] = sv >= len ? 0.0f
+                      : sv <= kHalo ? (float)win[(sx - wx0) * kWin + (sy - wy0)]
+                                    : (float)elevs[sx * n + sy];
             }
           };
           load_group(1);
@@ -187,17 +203,22 @@ siting_status launch_vix(const Geometry &g, const uint16_t *elevs, uint8_t *vix,
   const int side = (n + kTile - 1) / kTile;
   const int ntiles = side * side;
   const int per_xcd = (ntiles + 7) / 8;
-  const size_t lds = sizeof(uint32_t) * (4 * (size_t)g.p.ntests + kThreads);
-  // samples read per dependent step (tuning override SITING_VIX_GROUP)
-  int group = 1;
+  // samples read per dependent step and LDS halo (tuning overrides
+  // SITING_VIX_GROUP, SITING_VIX_HALO)
+  int group = 1, halo = 16;
   if (const char *e = std::getenv("SITING_VIX_GROUP")) group = std::atoi(e);
-#define SITING_VIX(G)                                                              \
-  hipLaunchKernelGGL(vix_kernel<G>, dim3(8 * per_xcd), dim3(kThreads), lds, s, elevs, \
-                     vix, t.recip, t.jump, n, g.p.roi, g.p.oht, g.p.tht, g.p.ntests, \
+  if (const char *e = std::getenv("SITING_VIX_HALO")) halo = std::atoi(e);
+  const size_t base = sizeof(uint32_t) * (4 * (size_t)g.p.ntests + kThreads);
+#define SITING_VIX(G, H)                                                             \
+  hipLaunchKernelGGL((vix_kernel<G, H>), dim3(8 * per_xcd), dim3(kThreads),             \
+                     base + sizeof(uint16_t) * win_edge<H>() * win_edge<H>(), s, elevs, \
+                     vix, t.recip, t.jump, n, g.p.roi, g.p.oht, g.p.tht, g.p.ntests,   \
                      side, per_xcd)
-  if (group == 4) SITING_VIX(4);
-  else if (group == 2) SITING_VIX(2);
-  else SITING_VIX(1);
+  if (group == 4) SITING_VIX(4, 16);
+  else if (group == 2) SITING_VIX(2, 16);
+  else if (halo == 8) SITING_VIX(1, 8);
+  else if (halo == 32) SITING_VIX(1, 32);
+  else SITING_VIX(1, 16);
 #undef SITING_VIX
   SITING_HIP(hipGetLastError());
   return SITING_OK;
