@@ -250,7 +250,10 @@ struct Winner {
 // of the shed (rows are contiguous).  kBatch items per lane are loaded
 // before any is used.
 constexpr int kBatch = 4;
+// items per lane in flight when re-evaluating (8 spills and measured slower)
+constexpr int kEvalBatch = 4;
 
+template <int kB = kBatch>
 __device__ __forceinline__ int lane_new_bits(const uint64_t *sh, int cx, int cy,
                                              const SiteArgs &a, const Winner &W,
                                              int gl, int gs, bool use_w) {
@@ -268,10 +271,10 @@ __device__ __forceinline__ int lane_new_bits(const uint64_t *sh, int cx, int cy,
   const bool fdiv = items < (1 << 21);
   const int xo = cx - roi, dro = xo - W.wx + roi;
   int s = 0;
-  for (int base = 0; base < items; base += gs * kBatch) {
-    uint64_t cur[kBatch], prv[kBatch], cv[kBatch];
+  for (int base = 0; base < items; base += gs * kB) {
+    uint64_t cur[kB], prv[kB], cv[kB];
 #pragma unroll
-    for (int u = 0; u < kBatch; u++) {
+    for (int u = 0; u < kB; u++) {
       const int it = base + u * gs + gl;
       const bool ok = it < items;
       const int q = fdiv ? (int)(((float)it + 0.5f) * inv_nk) : it / nk;
@@ -286,7 +289,7 @@ __device__ __forceinline__ int lane_new_bits(const uint64_t *sh, int cx, int cy,
       cv[u] = c;
     }
 #pragma unroll
-    for (int u = 0; u < kBatch; u++) {
+    for (int u = 0; u < kB; u++) {
       uint64_t moved = cur[u] >> fbc;
       if (fbc != 0) moved |= prv[u] << (64 - fbc);
       s += __popcll(moved & ~cv[u]);
@@ -880,7 +883,8 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           const uint32_t ps = P[i];
           const int cx = (int)(ps >> 16), cy = (int)(ps & 0xffffu);
           const uint64_t *sh = a.sheds + (size_t)C[i] * S;
-          const int s = wave_sum_i32(lane_new_bits(sh, cx, cy, a, W0, gl, gw * kWave, false));
+          const int s =
+              wave_sum_i32(lane_new_bits<kEvalBatch>(sh, cx, cy, a, W0, gl, gw * kWave, false));
           if (lane == 0) atomicAdd(&list_acc[e], s);
         }
         __syncthreads();
