@@ -209,6 +209,10 @@ siting_status siting_batch_sync(siting_batch *b);
 siting_status siting_batch_elapsed_ms(siting_batch *b, float *ms);
 /* Device time of the last run's SITE work-queue launch alone (after sync). */
 siting_status siting_batch_site_ms(siting_batch *b, float *ms);
+/* Start and end of every job's SITE loop in the last run (after sync), in
+ * ms from the first loop's start, on the device clock: start_end[2*i],
+ * start_end[2*i+1] for job i (njobs pairs). */
+siting_status siting_batch_site_times(siting_batch *b, double *start_end, int njobs);
 /* Borrowed handle of job i, for downloads and stats (valid until destroy). */
 siting_status siting_batch_job(siting_batch *b, int job, siting_job **out);
 

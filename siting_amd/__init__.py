@@ -118,6 +118,7 @@ _SIGS = {
     "siting_batch_sync": (_I, [_VP]),
     "siting_batch_elapsed_ms": (_I, [_VP, ctypes.POINTER(ctypes.c_float)]),
     "siting_batch_site_ms": (_I, [_VP, ctypes.POINTER(ctypes.c_float)]),
+    "siting_batch_site_times": (_I, [_VP, _VP, _I]),
     "siting_batch_job": (_I, [_VP, _I, ctypes.POINTER(_VP)]),
     "siting_server_create": (_I, [_I, _VP, ctypes.POINTER(_VP)]),
     "siting_server_destroy": (None, [_VP]),
@@ -400,6 +401,13 @@ class Batch:
         ms = ctypes.c_float()
         _check(lib().siting_batch_site_ms(self._h, ctypes.byref(ms)))
         return ms.value
+
+    def site_times(self) -> np.ndarray:
+        """(njobs, 2) start and end of every job's SITE loop in the last run,
+        ms from the first loop's start."""
+        out = np.zeros(2 * len(self.params), np.float64)
+        _check(lib().siting_batch_site_times(self._h, _ptr(out), len(self.params)))
+        return out.reshape(-1, 2)
 
     def job(self, i) -> Job:
         h = ctypes.c_void_p()

@@ -973,6 +973,20 @@ siting_status siting_batch_site_ms(siting_batch *b, float *ms) {
   return SITING_OK;
 }
 
+siting_status siting_batch_site_times(siting_batch *b, double *start_end, int njobs) {
+  if (!b || !start_end || njobs != (int)b->jobs.size()) return SITING_EINVAL;
+  SITING_HIP(hipSetDevice(b->device));
+  SITING_HIP(hipEventSynchronize(b->t1));
+  std::vector<unsigned long long> t((size_t)2 * njobs);
+  SITING_HIP(hipMemcpy(t.data(), b->jobt, sizeof(unsigned long long) * t.size(),
+                       hipMemcpyDeviceToHost));
+  unsigned long long t0 = ~0ull;
+  for (int i = 0; i < njobs; i++) t0 = t[(size_t)2 * i] < t0 ? t[(size_t)2 * i] : t0;
+  for (size_t k = 0; k < t.size(); k++)
+    start_end[k] = t[k] >= t0 ? (double)(t[k] - t0) * 1e-5 : 0.0;  // 100 MHz clock
+  return SITING_OK;
+}
+
 siting_status siting_batch_job(siting_batch *b, int job, siting_job **out) {
   if (!b || !out || job < 0 || job >= (int)b->jobs.size()) return SITING_EINVAL;
   *out = b->jobs[(size_t)job];
