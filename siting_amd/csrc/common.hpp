@@ -159,8 +159,7 @@ struct SiteArgs {
   int n, roi, nr, nwpr, cw, nsheds, intervis;
   int cs, ncr, Q, nunits, G, upw, kpu, nbuckets, cap;
   // dynamic LDS layout (byte offsets)
-  int o_delta, o_ukey, o_upos, o_dirty, o_off, o_c, o_pos, o_t, o_used, o_sk, o_sp,
-      o_stk, o_seen;
+  int o_delta, o_ukey, o_upos, o_dirty, o_off, o_c, o_pos, o_t, o_used, o_seen;
   int off_lds;    // unit cell offsets kept in LDS (else read from bstart)
   int delta_lds;  // the winner's shifted shed staged in LDS (else recomputed)
   int tau8;       // intervis 0: update candidates whose area is below tau8/8

@@ -72,7 +72,11 @@ constexpr int kMaxList = kLoopThreads;  // list windows: one item per thread
 constexpr size_t kLdsBudget = 132 * 1024;  // dynamic LDS of the loop kernel
 constexpr int kMaxGroups = 128;
 constexpr int kMaxBatch = kSiteMaxBatch;
-
+// Wavefronts that gather the published unit keys for the pick walk, and the
+// position bit (bit 15: y < 32768) that flags a key whose successor in its
+// unit or lane list is unknown.
+constexpr int kPickWaves = 4;
+constexpr uint32_t kPickLast = 0x8000u;
 
 
 __device__ __forceinline__ uint64_t cand_key(int area, int c) {
@@ -390,6 +394,34 @@ __device__ __forceinline__ uint64_t wave_max_u64_dpp(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// The three largest (key, position) pairs pushed, in descending order; keys
+// are distinct and 0 marks an empty slot.
+struct Top3 {
+  uint64_t k[3];
+  uint32_t p[3];
+};
+
+__device__ __forceinline__ void top3_push(Top3 &t, uint64_t k, uint32_t p) {
+  if (k > t.k[2]) {
+    if (k > t.k[1]) {
+      t.k[2] = t.k[1];
+      t.p[2] = t.p[1];
+      if (k > t.k[0]) {
+        t.k[1] = t.k[0];
+        t.p[1] = t.p[0];
+        t.k[0] = k;
+        t.p[0] = p;
+      } else {
+        t.k[1] = k;
+        t.p[1] = p;
+      }
+    } else {
+      t.k[2] = k;
+      t.p[2] = p;
+    }
+  }
+}
+
 // Best and second best (key, position) over the unused candidates of owned
 // unit ul; one wavefront.
 struct Top2 {
@@ -437,6 +469,8 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   __shared__ uint32_t bw_pos[kMaxBatch];
   __shared__ int bw_n;
   __shared__ int bw_head;
+  __shared__ uint64_t xk[3 * (kPickWaves - 1)][kWave];  // pick lists of wavefronts 1..
+  __shared__ uint32_t xp[3 * (kPickWaves - 1)][kWave];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wid = tid >> 6;
@@ -453,9 +487,6 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   uint32_t *upos = (uint32_t *)(smem + a.o_upos);   // [upw][2]
   uint8_t *dirty = smem + a.o_dirty;                 // [upw]
   int32_t *off_l = (int32_t *)(smem + a.o_off);
-  uint64_t *sk = (uint64_t *)(smem + a.o_sk);       // [nunits][2] staged keys
-  uint32_t *sp = (uint32_t *)(smem + a.o_sp);       // [nunits][2] staged positions
-  uint8_t *stk = smem + a.o_stk;                     // [nunits] picks taken
 
   // ---- this workgroup's candidates
   const int B0 = a.bstart[(size_t)g * upw * kpu];
@@ -534,90 +565,95 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   long long cumarea = 0, my_u = 0;
 
   for (;;) {
-    // ---- stage the unit tops and pick the round's winners (identical in
-    // every workgroup).  Winners are a prefix of the global key order in
+    // ---- pick the round's winners (identical in every workgroup).  Winners
+    // are a prefix of the descending order of all published unit keys in
     // which no two are within 2*roi of each other: none of them is in
-    // another's update list, so the sequential loop would pick them in
-    // this order (areas only shrink when intervis is 0).  A unit whose two
-    // published candidates are both taken ends the prefix (its third is
-    // unknown).
+    // another's update list, so the sequential loop would pick them in this
+    // order (areas only shrink when intervis is 0).  A unit's second key
+    // sorts below its first, so the order needs no per-unit state; taking a
+    // unit's second key ends the prefix, since its third is unknown.
+    //
+    // Lane l of wavefronts 0..3 keeps the three largest keys of units
+    // l, l + 64, ... (strided by 256) in registers; wavefronts 1..3 hand
+    // theirs to wavefront 0 through LDS, whose lanes then hold the three
+    // largest of units l + 64 j.  A lane's third key is flagged like a
+    // second key: taking it ends the prefix, as the lane's fourth is unknown.
     {
       const uint64_t *kk = a.keys + (size_t)parity * nunits * 2;
       const uint32_t *kp = a.kpos + (size_t)parity * nunits * 2;
-      for (int e = tid; e < 2 * nunits; e += kLoopThreads) {
-        sk[e] = kk[e];
-        sp[e] = kp[e];
-      }
-      for (int u = tid; u < nunits; u += kLoopThreads) stk[u] = 0;
-    }
-    __syncthreads();
-    if (wid == 0) {
-      // lane-local best over units lane, lane+64, ...: the unit's first key,
-      // or its second once the first is taken this round (or the taken
-      // second, as an upper bound of the unknown third)
-      // (units in chunks of 8 per lane, so that each chunk's LDS reads are
-      // in flight together)
-      auto lane_best = [&](uint64_t &lv, int &lu) {
-        lv = 0;
-        lu = -1;
-        for (int u0 = lane; u0 < nunits; u0 += 8 * kWave) {
-          int t[8];
-#pragma unroll
-          for (int j = 0; j < 8; j++) {
-            const int u = u0 + j * kWave;
-            t[j] = u < nunits ? stk[u] : 0;
-          }
-          uint64_t v[8];
-#pragma unroll
-          for (int j = 0; j < 8; j++) {
-            const int u = u0 + j * kWave;
-            v[j] = u < nunits ? sk[2 * u + (t[j] == 0 ? 0 : 1)] : 0;
-          }
-#pragma unroll
-          for (int j = 0; j < 8; j++)
-            if (v[j] > lv) {
-              lv = v[j];
-              lu = u0 + j * kWave;
-            }
+      Top3 h{};
+      if (wid < kPickWaves) {
+        for (int u0 = wid * kWave + lane; u0 < nunits; u0 += 2 * kPickWaves * kWave) {
+          const int u1 = u0 + kPickWaves * kWave;
+          const bool has1 = u1 < nunits;
+          const uint64_t a0 = kk[2 * u0], a1 = kk[2 * u0 + 1];
+          const uint32_t q0 = kp[2 * u0], q1 = kp[2 * u0 + 1];
+          const uint64_t b0 = has1 ? kk[2 * u1] : 0, b1 = has1 ? kk[2 * u1 + 1] : 0;
+          const uint32_t r0 = has1 ? kp[2 * u1] : 0, r1 = has1 ? kp[2 * u1 + 1] : 0;
+          top3_push(h, a0, q0);
+          top3_push(h, a1, q1 | kPickLast);
+          top3_push(h, b0, r0);
+          top3_push(h, b1, r1 | kPickLast);
         }
-      };
-      uint64_t lv;
-      int lu;
-      lane_best(lv, lu);
-      int k = 0;
-      int head = 0;  // area (or bound) of the largest key
-      while (k < kmax) {
-        const uint64_t bv = wave_max_u64_dpp(lv);
-        if ((bv >> 32) == 0) break;       // no candidate adds area
-        if (k == 0) head = (int)(bv >> 32);
-        // the lane that holds it (keys are distinct)
-        const uint64_t hit = __ballot(lv == bv);
-        const int bu = __shfl(lu, (int)__builtin_ctzll(hit), kWave);
-        const int t = stk[bu];
-        if (t >= 2) break;                // the unit's next is unknown
-        const uint32_t pos = sp[2 * bu + t];
-        if (pos >> 31) break;             // stale: its area is only a bound
-        const int px = (int)(pos >> 16), py = (int)(pos & 0xffffu);
-        bool near = false;
+        if (wid > 0) {
+          const int s = (wid - 1) * 3;
+          xk[s][lane] = h.k[0];
+          xk[s + 1][lane] = h.k[1];
+          xk[s + 2][lane] = h.k[2];
+          xp[s][lane] = h.p[0];
+          xp[s + 1][lane] = h.p[1];
+          xp[s + 2][lane] = h.p[2];
+        }
+      }
+      __syncthreads();
+      if (wid == 0) {
+#pragma unroll
+        for (int s = 0; s < 3 * (kPickWaves - 1); s++) top3_push(h, xk[s][lane], xp[s][lane]);
+        if (h.k[2] != 0) h.p[2] |= kPickLast;
+        // Lane j < k holds pick j.
+        uint64_t pk = 0;
+        uint32_t pp = 0;
+        int k = 0;
+        int head = 0;  // area (or bound) of the largest key
+        while (k < kmax) {
+          const uint64_t bv = wave_max_u64_dpp(h.k[0]);
+          if ((bv >> 32) == 0) break;     // no candidate adds area
+          if (k == 0) head = (int)(bv >> 32);
+          // the lane that holds it (keys are distinct)
+          const int src = (int)__builtin_ctzll(__ballot(h.k[0] == bv));
+          const uint32_t pos = (uint32_t)__builtin_amdgcn_readlane((int)h.p[0], src);
+          if (pos >> 31) break;           // stale: its area is only a bound
+          const int px = (int)(pos >> 16), py = (int)(pos & 0x7fffu);
+          bool near = false;
+          if (lane < k) {
+            const int ex = px - (int)(pp >> 16);
+            const int ey = py - (int)(pp & 0xffffu);
+            near = ex * ex + ey * ey <= r2;
+          }
+          if (__any(near)) break;
+          if (lane == k) {
+            pk = bv;
+            pp = pos & ~kPickLast;
+          }
+          if (lane == src) {
+            h.k[0] = h.k[1];
+            h.p[0] = h.p[1];
+            h.k[1] = h.k[2];
+            h.p[1] = h.p[2];
+            h.k[2] = 0;
+            h.p[2] = 0;
+          }
+          k++;
+          if (pos & kPickLast) break;     // the next key of its unit or lane is unknown
+        }
         if (lane < k) {
-          const int ex = px - (int)(bw_pos[lane] >> 16);
-          const int ey = py - (int)(bw_pos[lane] & 0xffffu);
-          near = ex * ex + ey * ey <= r2;
+          bw_key[lane] = pk;
+          bw_pos[lane] = pp;
         }
-        if (__any(near)) break;
         if (lane == 0) {
-          bw_key[k] = bv;
-          bw_pos[k] = pos;
+          bw_n = k;
+          bw_head = head;
         }
-        if ((bu & (kWave - 1)) == lane) {  // the lane that holds unit bu
-          stk[bu] = (uint8_t)(t + 1);
-          lane_best(lv, lu);
-        }
-        k++;
-      }
-      if (lane == 0) {
-        bw_n = k;
-        bw_head = head;
       }
     }
     __syncthreads();
@@ -980,8 +1016,7 @@ __global__ __launch_bounds__(kLoopThreads) void site_loop_queue(
 struct Plan {
   int cs, ncr, Q, nunits, G, upw, kpu, nbuckets, nchunks, cap, delta_lds, off_lds;
   size_t lds, nbwords;
-  int o_delta, o_ukey, o_upos, o_dirty, o_off, o_c, o_pos, o_t, o_used, o_sk, o_sp,
-      o_stk, o_seen;
+  int o_delta, o_ukey, o_upos, o_dirty, o_off, o_c, o_pos, o_t, o_used, o_seen;
 };
 
 // Column classes per cell row.  More classes spread a pick's update window
@@ -1044,16 +1079,10 @@ Plan make_plan(int n, int roi, int nsheds, int groups_cap, int fixed_groups = 0)
   size_t o = 0;
   p.o_delta = (int)o;
   if (p.delta_lds) o += delta_bytes;
-  p.o_sk = (int)o;
-  o += sizeof(uint64_t) * 2 * (size_t)p.nunits;
   p.o_ukey = (int)o;
   o += sizeof(uint64_t) * 2 * (size_t)p.upw;
-  p.o_sp = (int)o;
-  o += sizeof(uint32_t) * 2 * (size_t)p.nunits;
   p.o_upos = (int)o;
   o += sizeof(uint32_t) * 2 * (size_t)p.upw;
-  p.o_stk = (int)o;
-  o += (size_t)p.nunits;
   p.o_dirty = (int)o;
   o = align_up(o + (size_t)p.upw, 16);
   p.o_off = (int)o;
@@ -1175,9 +1204,6 @@ siting_status site_make_args(const Geometry &g, const int32_t *obs,
   a.o_ukey = p.o_ukey;
   a.o_upos = p.o_upos;
   a.o_dirty = p.o_dirty;
-  a.o_sk = p.o_sk;
-  a.o_sp = p.o_sp;
-  a.o_stk = p.o_stk;
   a.o_off = p.o_off;
   a.off_lds = p.off_lds;
   a.delta_lds = p.delta_lds;
