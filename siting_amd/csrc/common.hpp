@@ -165,7 +165,7 @@ struct SiteArgs {
   int tau8;       // intervis 0: update candidates whose area is below tau8/8
                   // of the round's largest become stale bounds instead of
                   // being decremented (0: never)
-  double npts;
+  long long stop_area;  // least cumulative area c with 100.0 * c / (n * n) > 95
 };
 
 // Device scratch SITE needs beyond SiteBuffers' arrays, and the size of the

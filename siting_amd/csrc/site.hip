@@ -76,6 +76,7 @@ constexpr int kMaxBatch = kSiteMaxBatch;
 // position bit (bit 15: y < 32768) that flags a key whose successor in its
 // unit or lane list is unknown.
 constexpr int kPickWaves = 4;
+static_assert(kMaxBatch <= kLoopWaves, "one wavefront per pick marks it used");
 constexpr uint32_t kPickLast = 0x8000u;
 
 
@@ -672,34 +673,42 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     // ---- bookkeeping in pick order; the 95% rule may cut the round short
     bool stop = false;
     int kc = k;  // picks whose update lists the sequential loop evaluates
-    for (int i = 0; i < k; i++) {
-      const int w = (int)(~(uint32_t)bw_key[i]);
-      nused++;
-      cumarea += (long long)(bw_key[i] >> 32);
-      if (g == 0 && tid == 0) {
+    {
+      uint64_t key[kMaxBatch];
+#pragma unroll
+      for (int i = 0; i < kMaxBatch; i++) key[i] = i < k ? bw_key[i] : 0;
+      const int nused0 = nused;
+#pragma unroll
+      for (int i = 0; i < kMaxBatch; i++) {
+        if (i >= k) break;
+        nused++;
+        cumarea += (long long)(key[i] >> 32);
+        if (cumarea >= a.stop_area) {  // cpp:601-604
+          stop = true;
+          k = i + 1;
+          kc = i;
+          break;
+        }
+        niter++;
+      }
+      if (g == 0 && tid < k) {
+        const int w = (int)(~(uint32_t)bw_key[tid]);
         a.used[w] = 1;
-        a.order[nused - 1] = w;
+        a.order[nused0 + tid] = w;
       }
-      if (100.0 * (double)cumarea / a.npts > 95.0) {  // cpp:601-604
-        stop = true;
-        k = i + 1;
-        kc = i;
-        break;
-      }
-      niter++;
     }
-    // owners mark the picks used
-    for (int i = 0; i < k; i++) {
-      const int w = (int)(~(uint32_t)bw_key[i]);
-      const int wx = (int)(bw_pos[i] >> 16), wy = (int)(bw_pos[i] & 0xffffu);
+    // owners mark the picks used: wavefront i takes pick i
+    if (wid < k) {
+      const int w = (int)(~(uint32_t)bw_key[wid]);
+      const int wx = (int)(bw_pos[wid] >> 16), wy = (int)(bw_pos[wid] & 0xffffu);
       const int w_r = wx / a.cs, w_cc = wy / a.cs;
       const int w_u = w_r * Q + w_cc % Q;
       if (w_u % G == g) {
         const int ul = w_u / G, kc2 = w_cc / Q;
         const int lo = OFF(ul, kc2), hi = OFF(ul, kc2 + 1);
-        for (int j = lo + tid; j < hi; j += kLoopThreads)
+        for (int j = lo + lane; j < hi; j += kWave)
           if (C[j] == w) U[j] = 1;
-        if (tid == 0) dirty[ul] = 1;
+        if (lane == 0) dirty[ul] = 1;
       }
     }
 
@@ -974,7 +983,7 @@ __global__ __launch_bounds__(kLoopThreads) void site_loop(SiteArgs a) {
 // site_queue_plan_ints) and run each job's loop with the slot's first G
 // workgroups; jobt[j] receives the loop's start and end (device clock).
 __global__ __launch_bounds__(kLoopThreads) void site_loop_queue(
-    const SiteArgs *jobs, const int *plan, unsigned int *ctl,
+    const SiteArgs *__restrict__ jobs, const int *plan, unsigned int *ctl,
     unsigned long long *jobt) {
   __shared__ int job;
   const int nslots = plan[0], ncls = plan[1];
@@ -999,7 +1008,8 @@ __global__ __launch_bounds__(kLoopThreads) void site_loop_queue(
     __syncthreads();
     const int idx = job;
     if (idx >= qlen) break;
-    const int j = order[off[c] + idx];
+    // a wave-uniform job index lets the job's arguments be scalar loads
+    const int j = __builtin_amdgcn_readfirstlane(order[off[c] + idx]);
     const SiteArgs &a = jobs[j];
     const unsigned long long t0 = wall_clock64();
     if (g < a.G) site_loop_body(a, g, a.G);
@@ -1053,6 +1063,22 @@ int cell_edge(int n, int roi) {
 }
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// The stop rule 100.0 * cumarea / (n * n) > 95 (siting.cpp:601-604), in
+// double arithmetic, is monotone in cumarea: the least area that meets it,
+// or n * n + 1 when none does.
+long long stop_area(int n) {
+  const double npts = (double)n * (double)n;
+  long long lo = 0, hi = (long long)n * n + 1;  // the answer is in [lo, hi]
+  while (lo < hi) {
+    const long long mid = lo + (hi - lo) / 2;
+    if (100.0 * (double)mid / npts > 95.0)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
 
 Plan make_plan(int n, int roi, int nsheds, int groups_cap, int fixed_groups = 0) {
   Plan p{};
@@ -1213,7 +1239,7 @@ siting_status site_make_args(const Geometry &g, const int32_t *obs,
   a.o_used = p.o_used;
   a.o_seen = p.o_seen;
   a.tau8 = tau8();
-  a.npts = (double)n * (double)n;
+  a.stop_area = stop_area(n);
   *out = a;
   if (lds) *lds = p.lds;
   return SITING_OK;
