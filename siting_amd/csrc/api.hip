@@ -68,7 +68,7 @@ using namespace siting;
 namespace {
 
 struct Layout {
-  size_t elevs, vix, obs, sheds, area, used, cum, rowmax, order, state, recip,
+  size_t elevs, elevs_t, vix, obs, sheds, area, used, cum, rowmax, order, state, recip,
       jump, rdiv, scratch, total;
 };
 
@@ -84,6 +84,7 @@ Layout layout_for(const siting_params &p, const siting_derived &d) {
     off += round_up(bytes ? bytes : 1);
   };
   put(L.elevs, n2 * 2);
+  put(L.elevs_t, n2 * 2);
   put(L.vix, n2);
   put(L.obs, nw * 4);
   put(L.sheds, nw * (size_t)d.shed_words * 8);
@@ -211,6 +212,7 @@ siting_status job_create_in(int device, const siting_params *p, void *mem,
   }
   uint8_t *b = (uint8_t *)j->base;
   j->elevs = (uint16_t *)(b + L.elevs);
+  j->elevs_t = (uint16_t *)(b + L.elevs_t);
   j->vix = b + L.vix;
   j->obs = (int32_t *)(b + L.obs);
   j->sheds = (uint64_t *)(b + L.sheds);
@@ -300,7 +302,8 @@ siting_status siting_job_findmax(siting_job *j) {
   JOB_STAGE(1, launch_findmax(j->g, j->vix, j->obs, j->stream));
 }
 siting_status siting_job_viewshed(siting_job *j) {
-  JOB_STAGE(2, launch_viewshed(j->g, j->elevs, j->obs, j->sheds, j->rdiv, j->stream));
+  JOB_STAGE(2, launch_viewshed(j->g, j->elevs, j->elevs_t, j->obs, j->sheds, j->rdiv,
+                               j->stream));
 }
 siting_status siting_job_site(siting_job *j) {
   JOB_STAGE(3, launch_site(j->g, j->obs, j->sheds, j->site, j->device, j->stream,
@@ -511,8 +514,9 @@ siting_status siting_calc_sheds(int nrows, const uint16_t *elevs, int roi, int o
   g.d.nr = 2 * roi + 1;
   g.d.nwpr = (g.d.nr + 63) / 64;
   const size_t sbytes = (size_t)nsheds * g.d.nr * g.d.nwpr * 8;
-  DevBuf de, dobs, dsh, drd;
+  DevBuf de, det, dobs, dsh, drd;
   CHECK(dalloc(de, n2 * 2));
+  CHECK(dalloc(det, n2 * 2));
   CHECK(dalloc(dobs, (size_t)nsheds * 4));
   CHECK(dalloc(dsh, sbytes));
   CHECK(dalloc(drd, rdiv_entries(roi) * 8));
@@ -521,7 +525,7 @@ siting_status siting_calc_sheds(int nrows, const uint16_t *elevs, int roi, int o
   SITING_HIP(hipMemcpyAsync(de.p, elevs, n2 * 2, hipMemcpyHostToDevice, s));
   SITING_HIP(hipMemcpyAsync(dobs.p, obs, (size_t)nsheds * 4, hipMemcpyHostToDevice, s));
   SITING_HIP(hipMemcpyAsync(drd.p, rdv.data(), rdv.size() * 8, hipMemcpyHostToDevice, s));
-  CHECK(launch_viewshed(g, (const uint16_t *)de.p, (const int32_t *)dobs.p,
+  CHECK(launch_viewshed(g, (const uint16_t *)de.p, (uint16_t *)det.p, (const int32_t *)dobs.p,
                         (uint64_t *)dsh.p, (const double *)drd.p, s));
   SITING_HIP(hipMemcpyAsync(sheds, dsh.p, sbytes, hipMemcpyDeviceToHost, s));
   SITING_HIP(hipStreamSynchronize(s));
@@ -679,7 +683,8 @@ siting_status batch_enqueue_prep(siting_batch *b) {
     if ((st = launch_vix(j->g, j->elevs, j->vix, VixTables{j->recip, j->jump}, s)) != SITING_OK)
       return st;
     if ((st = launch_findmax(j->g, j->vix, j->obs, s)) != SITING_OK) return st;
-    if ((st = launch_viewshed(j->g, j->elevs, j->obs, j->sheds, j->rdiv, s)) != SITING_OK)
+    if ((st = launch_viewshed(j->g, j->elevs, j->elevs_t, j->obs, j->sheds, j->rdiv, s)) !=
+        SITING_OK)
       return st;
     SiteArgs a;
     if ((st = site_make_args(j->g, j->obs, j->sheds, j->site, b->cap[(size_t)i], &a,

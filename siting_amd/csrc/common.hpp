@@ -121,7 +121,9 @@ siting_status launch_findmax(const Geometry &g, uint8_t *vix, int32_t *obs,
 // at a time, and a scalar load under a lane-divergent condition may run for
 // a step that no lane takes (at most three past roi).
 inline size_t rdiv_entries(int roi) { return (size_t)roi + 4; }
-siting_status launch_viewshed(const Geometry &g, const uint16_t *elevs,
+// elevs_t: n * n scratch that receives the DEM transposed (the rays whose
+// major axis is y read it, so that a wavefront's reads share rows).
+siting_status launch_viewshed(const Geometry &g, const uint16_t *elevs, uint16_t *elevs_t,
                               const int32_t *obs, uint64_t *sheds,
                               const double *rdiv, hipStream_t s);
 

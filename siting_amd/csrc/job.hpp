@@ -10,6 +10,7 @@ struct siting_job {
   void *base = nullptr;
   size_t bytes = 0;
   uint16_t *elevs = nullptr;
+  uint16_t *elevs_t = nullptr;  // the DEM transposed, written by VIEWSHED
   uint8_t *vix = nullptr;
   int32_t *obs = nullptr;
   uint64_t *sheds = nullptr;

@@ -20,6 +20,7 @@
 // takes a contiguous run of observers, which FINDMAX emitted block by block,
 // i.e. spatially adjacent), so each XCD's L2 serves one band of the DEM.
 #include <cstdlib>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -62,6 +63,23 @@ __device__ __forceinline__ uint64_t wave_transpose64(uint64_t x, int lane) {
   return x;
 }
 
+__device__ __forceinline__ int wave_min_i32(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(v, off, kWave);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+// The major axis a wavefront's rays walk, in one direction (kAxisMixed:
+// each lane's own).
+enum { kAxisY = 0, kAxisX = 1, kAxisMixed = 2 };
+
+// 0.5 - 2^-25, the largest float below one half: trunc(fl(x + kHalfDown))
+// rounds a non-negative float x below 2^15 half away from zero.
+constexpr float kHalfDown = 0.49999997f;
+
 __device__ __forceinline__ int wave_max_i32(int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -73,7 +91,8 @@ __device__ __forceinline__ int wave_max_i32(int v) {
 
 template <int kMode>
 __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
-    const uint16_t *__restrict__ elevs, const int32_t *__restrict__ obs,
+    const uint16_t *__restrict__ elevs, const uint16_t *__restrict__ elevs_t,
+    const int32_t *__restrict__ obs,
     uint64_t *__restrict__ sheds, const double *__restrict__ rd, int n, int roi,
     int oht, int tht, int nwpr, int nsheds, int per_xcd) {
   const int nthreads = blockDim.x;
@@ -98,49 +117,49 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
   const int oalt = (int)elevs[o] + oht;
   const float foalt = (float)oalt;
   const int rr = roi * roi;
-  const int twor = 2 * roi;
 
   if (threadIdx.x == 0)  // observer sees itself, siting.cpp:279
     atomicOr((unsigned long long *)&bits[roi * (kMode == kSplit ? ps : nwpr) + (roi >> 6)],
              1ull << (63 - (roi & 63)));
 
-  // Rays are dealt in passes of nthreads; every lane of a pass takes part
-  // (lanes past the last ray walk no step), so that each wavefront walks
-  // its rays' steps together: the step index is then uniform across the
-  // wavefront and 1.0/step comes from the table through scalar loads.
+  // Rays are dealt in passes of nthreads, the y-major ones first (the
+  // interior cells of the rows dy = -roi and dy = roi), then the x-major ones
+  // (the columns dx = -roi and dx = roi, corners included), so that every
+  // wavefront but one walks a single major axis.  Lanes past the last ray
+  // repeat it, OR'ing the same bits.  A wavefront walks its rays' steps
+  // together: the step index is uniform across it and 1.0/step comes from
+  // the table through scalar loads.
+  const int ny = 4 * roi - 2;
   for (int base = 0; base < 8 * roi; base += nthreads) {
-    const bool live = base + (int)threadIdx.x < 8 * roi;
-    const int ray = live ? base + (int)threadIdx.x : 0;
-    const int side = ray / twor;
-    const int j = ray - side * twor;
+    const int ray = min(base + (int)threadIdx.x, 8 * roi - 1);
     int dx, dy;
-    if (side == 0) {
-      dx = -roi + j;
-      dy = -roi;
-    } else if (side == 1) {
-      dx = -roi + 1 + j;
-      dy = roi;
-    } else if (side == 2) {
-      dx = -roi;
-      dy = -roi + 1 + j;
+    if (ray < ny) {
+      const bool lo = ray < 2 * roi - 1;
+      dx = -roi + 1 + (lo ? ray : ray - (2 * roi - 1));
+      dy = lo ? -roi : roi;
     } else {
-      dx = roi;
-      dy = -roi + j;
+      const int r = ray - ny;
+      const bool lo = r < 2 * roi + 1;
+      dx = lo ? -roi : roi;
+      dy = -roi + (lo ? r : r - (2 * roi + 1));
     }
-    const int adx = dx < 0 ? -dx : dx, ady = dy < 0 ? -dy : dy;
-    const bool along_y = adx < ady;
+    const bool along_y = ray < ny;  // |dx| < |dy| exactly for these
     const int major = along_y ? dy : dx;
     const int minor = along_y ? dx : dy;
-    const int len = along_y ? ady : adx;
+    const int len = major < 0 ? -major : major;
     const int dir = major > 0 ? 1 : -1;
     const float slope = (float)minor / (float)major;  // siting.cpp:323
-    // cell of step a (siting.cpp:330): the minor offset is the rounded float
-    // product, as the reference computes it
-    auto cell = [&](int a, int &qx, int &qy) {
-      const int i = dir * a;
-      const int off = round_half_away((float)i * slope);
-      qx = along_y ? off : i;
-      qy = along_y ? i : off;
+    // The minor offset of step a is fl(i * slope) rounded half away from
+    // zero, i = dir * a (siting.cpp:330).  Its magnitude is fl(a * |slope|)
+    // rounded the same way, which for every float below 2^15 equals
+    // trunc(fl(x + (0.5 - 2^-25))) (checked exhaustively); its sign is that
+    // of dir * slope.
+    const float aslope = fabsf(slope);
+    const int sgn = (dir > 0) == (slope >= 0.0f) ? 1 : -1;
+    auto offset_abs = [&](float fa) {
+      const int r = (int)__fadd_rn(__fmul_rn(fa, aslope), kHalfDown);
+      __builtin_assume(r >= 0 && r < 32768);  // |offset| <= a <= roi < 2^14
+      return r;
     };
     // The reference walks a = 1..len and stops at the first cell outside the
     // terrain or the roi disk (siting.cpp:333-334).  Both tests are monotone
@@ -149,72 +168,114 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
     // monotone in i).  The steps taken are therefore 1..last, with last
     // found by bisection; the walk below then needs no exit tests.
     int last = 0;
-    if (live) {
+    {
       int hi = len + 1;  // cell(last) inside (a = 0 is the observer), cell(hi) not
       while (hi - last > 1) {
         const int mid = (last + hi) >> 1;
-        int qx, qy;
-        cell(mid, qx, qy);
+        const int i = dir * mid, off = sgn * offset_abs((float)mid);
+        const int qx = along_y ? off : i, qy = along_y ? i : off;
         const int px = ox + qx, py = oy + qy;
         const bool in = px >= 0 && px < n && py >= 0 && py < n && qx * qx + qy * qy <= rr;
         if (in) last = mid; else hi = mid;
       }
     }
     float horizon = -99999.0f;  // siting.cpp:325
-    // siting.cpp:337-344 for one step.  The point slope is siting.cpp:340,
-    // (float)(pel - oalt) / (float)a: with |pel - oalt| < 2^21 and a < 2^14
-    // the quotient is never closer than 2^-39 (relative) to a float rounding
-    // midpoint, and never on one, while the double product below is within
-    // 2^-52 of it, so rounding it to float gives the correctly rounded fp32
-    // quotient, bit for bit.
-    // Lanes past their ray's last step compute too, and discard the result:
-    // the step stays branch-free up to the atomic, so the wait for the
-    // reciprocals happens once per group.
-    auto step = [&](bool take, int a, double rda, int pel, int qx, int qy) {
-      const float sl = (float)((double)(pel - oalt) * rda);
-      horizon = take && horizon < sl ? sl : horizon;
-      const float rise = horizon * (float)a;
-      const float halt = foalt + rise;  // siting.cpp:342
-      if (take && (float)(pel + tht) >= halt) {
-        const int row = qx + roi, col = qy + roi;
-        if (kMode == kSplit) {
-          // (24-bit products: rows, columns and strides are below 2^15)
-          const int w = along_y ? (int)__umul24(row, ps) + (col >> 6)
-                                : nr * ps + (int)__umul24(col, ps) + (row >> 6);
-          const int bit = along_y ? (col & 63) : (row & 63);
-          atomicOr((unsigned long long *)&bits[w], 1ull << (63 - bit));
-        } else {
-          atomicOr((unsigned long long *)&bits[row * nwpr + (col >> 6)],
-                   1ull << (63 - (col & 63)));
-        }
-      }
-    };
-    // four steps at a time, their DEM loads issued before any is used; the
-    // wavefront runs to its longest ray, each lane stepping up to its own
-    // The four reciprocals are read first (scalar loads share their wait
-    // counter with the LDS atomics, so a read just before its use would also
-    // wait for every atomic in flight; the table has three spare entries
-    // past roi for the last group).  DEM offsets are 32-bit with 24-bit
-    // products (n < 2^15).
+    const int wmin = __builtin_amdgcn_readfirstlane(wave_min_i32(last));
     const int wlast = __builtin_amdgcn_readfirstlane(wave_max_i32(last));
-    for (int a = 1; a <= wlast; a += 4) {
-      double rda[4];
+
+    // kCls: kAxisY or kAxisX when every lane of the wavefront walks that
+    // major axis in one direction (the step's major coordinate, the DEM row
+    // it reads and its bit position are then wavefront-uniform), else
+    // kAxisMixed (each lane selects).
+    auto walk = [&](auto cls_c) {
+      constexpr int kCls = decltype(cls_c)::value;
+      constexpr bool kUni = kCls != kAxisMixed;
+      const bool ay = kCls == kAxisY ? true : kCls == kAxisX ? false : along_y;
+      const int sdir = kUni ? __builtin_amdgcn_readfirstlane(dir) : dir;
+      const int sps = sgn * ps;
+      // siting.cpp:337-344 for one step.  The point slope is siting.cpp:340,
+      // (float)(pel - oalt) / (float)a: with |pel - oalt| < 2^21 and
+      // a < 2^14 the quotient is never closer than 2^-39 (relative) to a
+      // float rounding midpoint, and never on one, while the double product
+      // below is within 2^-52 of it, so rounding it to float gives the
+      // correctly rounded fp32 quotient, bit for bit.  (The running maximum
+      // compares as the reference's "if (horizon < s) horizon = s": the
+      // slopes are never NaN and the sign of a zero horizon changes
+      // nothing downstream.)
+      auto step = [&](bool take, int a, float fa, double rda, int pel, int offa) {
+        const float sl = (float)((double)(pel - oalt) * rda);
+        horizon = take ? fmaxf(horizon, sl) : horizon;
+        const float rise = __fmul_rn(horizon, fa);
+        const float halt = __fadd_rn(foalt, rise);  // siting.cpp:342
+        if (take && (float)(pel + tht) >= halt) {
+          // (24-bit products: rows, columns and strides are below 2^15)
+          const int c = roi + sdir * a;  // the major coordinate in the box
+          if (kMode == kSplit && kUni) {
+            // y-major: row off + roi, column c; x-major, transposed: column
+            // off + roi, row c.  One 32-bit half of the word takes the bit.
+            const int w = (ay ? roi * ps : nr * ps + roi * ps) + (c >> 6) + __mul24(offa, sps);
+            const int b = 63 - (c & 63);
+            atomicOr((unsigned int *)&bits[w] + (b >> 5), 1u << (b & 31));
+          } else {
+            const int off = __mul24(sgn, offa);
+            const int row = ay ? off + roi : c, col = ay ? c : off + roi;
+            if (kMode == kSplit) {
+              const int w = ay ? (int)__umul24(row, ps) + (col >> 6)
+                               : nr * ps + (int)__umul24(col, ps) + (row >> 6);
+              const int bit = ay ? (col & 63) : (row & 63);
+              atomicOr((unsigned long long *)&bits[w], 1ull << (63 - bit));
+            } else {
+              atomicOr((unsigned long long *)&bits[(int)__umul24(row, nwpr) + (col >> 6)],
+                       1ull << (63 - (col & 63)));
+            }
+          }
+        }
+      };
+      // Four steps at a time, their DEM loads issued before any is used.
+      // The four reciprocals are read first (scalar loads share their wait
+      // counter with the LDS atomics, so a read just before its use would
+      // also wait for every atomic in flight; the table has three spare
+      // entries past roi for the last group).  The y-major rays read the
+      // transposed DEM, so that a wavefront's reads share a row either way
+      // (in a uniform wavefront, the one row).  DEM offsets are 32-bit with
+      // 24-bit products (n < 2^15).
+      const uint16_t *dem = ay ? elevs_t : elevs;
+      const int omaj = ay ? oy : ox, omin = ay ? ox : oy;
+      auto group = [&](int a, auto checked_c) {
+        constexpr bool kChecked = decltype(checked_c)::value;
+        double rda[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) rda[u] = rd[a + u];
-      int qx[4], qy[4], pel[4];
+        for (int u = 0; u < 4; u++) rda[u] = rd[a + u];
+        float fa[4];
+        int offa[4], pel[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        cell(a + u, qx[u], qy[u]);
-        const uint32_t idx = __umul24((uint32_t)(ox + qx[u]), (uint32_t)n) + (uint32_t)(oy + qy[u]);
-        pel[u] = a + u <= last ? (int)elevs[idx] : 0;
-      }
-      // (no instruction: makes the four reciprocals ready here, after the
-      // DEM loads are issued and before the steps' conditional blocks, so
-      // that none of those waits again)
-      asm volatile("" ::"s"(rda[0]), "s"(rda[1]), "s"(rda[2]), "s"(rda[3]));
+        for (int u = 0; u < 4; u++) {
+          fa[u] = (float)(a + u);
+          offa[u] = offset_abs(fa[u]);
+          const uint32_t prow = (uint32_t)(omaj + sdir * (a + u));
+          const uint32_t pcol = (uint32_t)(omin + __mul24(sgn, offa[u]));
+          pel[u] = !kChecked || a + u <= last ? (int)dem[__umul24(prow, (uint32_t)n) + pcol] : 0;
+        }
+        // (no instruction: makes the four reciprocals ready here, after the
+        // DEM loads are issued and before the steps' conditional blocks, so
+        // that none of those waits again)
+        asm volatile("" ::"s"(rda[0]), "s"(rda[1]), "s"(rda[2]), "s"(rda[3]));
 #pragma unroll
-      for (int u = 0; u < 4; u++) step(a + u <= last, a + u, rda[u], pel[u], qx[u], qy[u]);
-    }
+        for (int u = 0; u < 4; u++)
+          step(!kChecked || a + u <= last, a + u, fa[u], rda[u], pel[u], offa[u]);
+      };
+      int a = 1;
+      // steps every lane takes, then the rest, each lane up to its own last
+      for (; a + 3 <= wmin; a += 4) group(a, std::integral_constant<bool, false>{});
+      for (; a <= wlast; a += 4) group(a, std::integral_constant<bool, true>{});
+    };
+    const int cls = (along_y ? 0 : 2) + (dir > 0 ? 1 : 0);
+    if (__ballot(cls == __builtin_amdgcn_readfirstlane(cls)) != ~0ull)
+      walk(std::integral_constant<int, kAxisMixed>{});
+    else if (along_y)
+      walk(std::integral_constant<int, kAxisY>{});
+    else
+      walk(std::integral_constant<int, kAxisX>{});
   }
 
   if (kMode == kLds) {
@@ -240,6 +301,26 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
   }
 }
 
+// elevs_t[y * n + x] = elevs[x * n + y], through 64 x 64 LDS tiles: rows of
+// a tile are read and columns written with consecutive lanes on consecutive
+// cells.
+constexpr int kTile = 64, kTileRows = 4;
+__global__ __launch_bounds__(kTile * kTileRows) void dem_transpose(
+    const uint16_t *__restrict__ in, uint16_t *__restrict__ out, int n) {
+  __shared__ uint16_t tile[kTile][kTile + 2];
+  const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
+  const int x0 = blockIdx.y * kTile, y0 = blockIdx.x * kTile;
+  for (int r = ty; r < kTile; r += kTileRows) {
+    const int x = x0 + r, y = y0 + tx;
+    if (x < n && y < n) tile[r][tx] = in[(size_t)x * n + y];
+  }
+  __syncthreads();
+  for (int r = ty; r < kTile; r += kTileRows) {
+    const int y = y0 + r, x = x0 + tx;
+    if (x < n && y < n) out[(size_t)y * n + x] = tile[tx][r];
+  }
+}
+
 }  // namespace
 
 void fill_rdiv_table(int roi, double *out) {
@@ -247,11 +328,14 @@ void fill_rdiv_table(int roi, double *out) {
   for (int a = 1; a <= roi; a++) out[a] = 1.0 / (double)a;
 }
 
-siting_status launch_viewshed(const Geometry &g, const uint16_t *elevs,
+siting_status launch_viewshed(const Geometry &g, const uint16_t *elevs, uint16_t *elevs_t,
                               const int32_t *obs, uint64_t *sheds,
                               const double *rdiv, hipStream_t s) {
   const int nsheds = g.d.nwanted;
   if (nsheds <= 0) return SITING_OK;
+  const int tiles = (g.p.nrows + kTile - 1) / kTile;
+  hipLaunchKernelGGL(dem_transpose, dim3(tiles, tiles), dim3(kTile * kTileRows), 0, s, elevs,
+                     elevs_t, g.p.nrows);
   const int per_xcd = (nsheds + 7) / 8;
   const int words = g.d.nr * g.d.nwpr;
   // 256 threads (4 wavefronts) per observer, rays dealt round-robin; the
@@ -267,8 +351,8 @@ siting_status launch_viewshed(const Geometry &g, const uint16_t *elevs,
                      : mode == kLds ? sizeof(uint64_t) * words : 0;
 #define SITING_VS(A)                                                               \
   hipLaunchKernelGGL((viewshed_kernel<A>), dim3(8 * per_xcd), dim3(threads), lds, s, \
-                     elevs, obs, sheds, rdiv, g.p.nrows, g.p.roi, g.p.oht, g.p.tht,   \
-                     g.d.nwpr, nsheds, per_xcd)
+                     elevs, elevs_t, obs, sheds, rdiv, g.p.nrows, g.p.roi, g.p.oht,    \
+                     g.p.tht, g.d.nwpr, nsheds, per_xcd)
   if (mode == kSplit) SITING_VS(kSplit);
   else if (mode == kLds) SITING_VS(kLds);
   else SITING_VS(kGlobal);

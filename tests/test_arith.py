@@ -52,18 +52,23 @@ def test_float_index_split_exact():
 
 def _ray_cells(roi, ray):
     """Perimeter target of ray `ray` and the ray's (major, minor, along_y, len,
-    dir, float32 slope), as VIEWSHED enumerates them (siting.cpp:292-323)."""
-    twor = 2 * roi
-    side, j = divmod(ray, twor)
-    if side == 0:
-        dx, dy = -roi + j, -roi
-    elif side == 1:
-        dx, dy = -roi + 1 + j, roi
-    elif side == 2:
-        dx, dy = -roi, -roi + 1 + j
+    dir, float32 slope), as VIEWSHED enumerates them (siting.cpp:292-323).
+    The kernel deals the y-major rays first (the interior cells of the rows
+    dy = -roi, dy = roi), then the x-major ones (the columns dx = -roi,
+    dx = roi, corners included): the same 8*roi perimeter cells as the
+    reference's four sides, each once."""
+    ny = 4 * roi - 2
+    if ray < ny:
+        lo = ray < 2 * roi - 1
+        dx = -roi + 1 + (ray if lo else ray - (2 * roi - 1))
+        dy = -roi if lo else roi
     else:
-        dx, dy = roi, -roi + j
+        r = ray - ny
+        lo = r < 2 * roi + 1
+        dx = -roi if lo else roi
+        dy = -roi + (r if lo else r - (2 * roi + 1))
     along_y = abs(dx) < abs(dy)
+    assert along_y == (ray < ny)
     major, minor = (dy, dx) if along_y else (dx, dy)
     slope = np.float32(minor) / np.float32(major)
     return along_y, abs(major), (1 if major > 0 else -1), slope
@@ -108,3 +113,46 @@ def test_viewshed_ray_prefix_by_bisection():
                             hi = mid
                     assert last == walk, (roi, n, ox, oy, ray)
                     assert ok[:last].all()
+
+
+def test_ray_set_is_the_box_perimeter():
+    """The kernel's ray order covers every perimeter cell of the (2roi+1)^2
+    box exactly once, as the reference's four sides do (siting.cpp:292-304)."""
+    for roi in (1, 2, 3, 25, 100, 150):
+        cells = []
+        for ray in range(8 * roi):
+            along_y, length, dirn, slope = _ray_cells(roi, ray)
+            end = _offsets(dirn, slope, np.array([length]))[0]
+            cells.append((int(end), dirn * length) if along_y else (dirn * length, int(end)))
+        want = {(x, y) for x in range(-roi, roi + 1) for y in range(-roi, roi + 1)
+                if max(abs(x), abs(y)) == roi}
+        assert len(cells) == len(set(cells)) == 8 * roi
+        assert set(cells) == want
+
+
+def test_half_away_rounding_by_biased_truncation():
+    """VIEWSHED rounds a ray's minor offset fl(i * slope) half away from zero
+    (siting.cpp:330) as sign * trunc(fl(fl(a * |slope|) + (0.5 - 2^-25))).
+    The magnitude identity floor(x + 0.5) == trunc(fl(x + 0.49999997f)) is
+    checked here for every float32 x in [0, 2^15) (offsets never exceed
+    roi < 2^14), and the whole offset against the reference formula for
+    every step of every ray of several roi."""
+    h = np.float32(0.49999997)
+    assert h.view(np.uint32) == 0x3EFFFFFF
+    top = int(np.float32(32768.0).view(np.uint32))
+    chunk = 1 << 25
+    for b0 in range(0, top, chunk):
+        x = np.arange(b0, min(top, b0 + chunk), dtype=np.uint32).view(np.float32)
+        got = (x + h).astype(np.int32)
+        want = np.floor(x.astype(np.float64) + 0.5).astype(np.int32)
+        bad = np.flatnonzero(got != want)
+        assert bad.size == 0, float(x[bad[0]])
+    for roi in (1, 2, 7, 25, 100, 150, 333):
+        for ray in range(8 * roi):
+            along_y, length, dirn, slope = _ray_cells(roi, ray)
+            a = np.arange(1, length + 1, dtype=np.int64)
+            ref = _offsets(dirn, slope, a)
+            sgn = 1 if (dirn > 0) == (slope >= 0) else -1
+            mag = (a.astype(np.float32) * np.float32(abs(slope))).astype(np.float32)
+            mine = sgn * (mag + h).astype(np.float32).astype(np.int64)
+            assert np.array_equal(mine, ref), (roi, ray)
