@@ -76,6 +76,14 @@ __device__ __forceinline__ int wave_min_i32(int v) {
 // each lane's own).
 enum { kAxisY = 0, kAxisX = 1, kAxisMixed = 2 };
 
+// max(x, y) as one v_max_f32 (fmaxf adds a canonicalising max of the
+// running value; no operand here is ever NaN)
+__device__ __forceinline__ float fmax_plain(float x, float y) {
+  float r;
+  asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(y));
+  return r;
+}
+
 // 0.5 - 2^-25, the largest float below one half: trunc(fl(x + kHalfDown))
 // rounds a non-negative float x below 2^15 half away from zero.
 constexpr float kHalfDown = 0.49999997f;
@@ -192,7 +200,7 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
       constexpr bool kUni = kCls != kAxisMixed;
       const bool ay = kCls == kAxisY ? true : kCls == kAxisX ? false : along_y;
       const int sdir = kUni ? __builtin_amdgcn_readfirstlane(dir) : dir;
-      const int sps = sgn * ps;
+      const int sps8 = sgn * ps * 8;
       // siting.cpp:337-344 for one step.  The point slope is siting.cpp:340,
       // (float)(pel - oalt) / (float)a: with |pel - oalt| < 2^21 and
       // a < 2^14 the quotient is never closer than 2^-39 (relative) to a
@@ -204,18 +212,19 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
       // nothing downstream.)
       auto step = [&](bool take, int a, float fa, double rda, int pel, int offa) {
         const float sl = (float)((double)(pel - oalt) * rda);
-        horizon = take ? fmaxf(horizon, sl) : horizon;
+        horizon = take ? fmax_plain(horizon, sl) : horizon;
         const float rise = __fmul_rn(horizon, fa);
         const float halt = __fadd_rn(foalt, rise);  // siting.cpp:342
         if (take && (float)(pel + tht) >= halt) {
           // (24-bit products: rows, columns and strides are below 2^15)
-          const int c = roi + sdir * a;  // the major coordinate in the box
+          // the major coordinate in the box (wavefront-uniform in kUni)
+          const int c = kUni ? __builtin_amdgcn_readfirstlane(roi + sdir * a) : roi + sdir * a;
           if (kMode == kSplit && kUni) {
             // y-major: row off + roi, column c; x-major, transposed: column
             // off + roi, row c.  One 32-bit half of the word takes the bit.
-            const int w = (ay ? roi * ps : nr * ps + roi * ps) + (c >> 6) + __mul24(offa, sps);
             const int b = 63 - (c & 63);
-            atomicOr((unsigned int *)&bits[w] + (b >> 5), 1u << (b & 31));
+            const int wb = 8 * ((ay ? roi * ps : nr * ps + roi * ps) + (c >> 6)) + 4 * (b >> 5);
+            atomicOr((unsigned int *)((uint8_t *)bits + wb + __mul24(offa, sps8)), 1u << (b & 31));
           } else {
             const int off = __mul24(sgn, offa);
             const int row = ay ? off + roi : c, col = ay ? c : off + roi;
@@ -241,20 +250,31 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
       // 24-bit products (n < 2^15).
       const uint16_t *dem = ay ? elevs_t : elevs;
       const int omaj = ay ? oy : ox, omin = ay ? ox : oy;
+      const int sgn2 = 2 * sgn;
       auto group = [&](int a, auto checked_c) {
         constexpr bool kChecked = decltype(checked_c)::value;
         double rda[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) rda[u] = rd[a + u];
-        float fa[4];
         int offa[4], pel[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-          fa[u] = (float)(a + u);
-          offa[u] = offset_abs(fa[u]);
-          const uint32_t prow = (uint32_t)(omaj + sdir * (a + u));
-          const uint32_t pcol = (uint32_t)(omin + __mul24(sgn, offa[u]));
-          pel[u] = !kChecked || a + u <= last ? (int)dem[__umul24(prow, (uint32_t)n) + pcol] : 0;
+          offa[u] = offset_abs((float)(a + u));
+          int v = 0;
+          if (!kChecked || a + u <= last) {
+            if (kUni) {
+              // the row's cell omin - roi as a scalar base, the lane's cell
+              // as a non-negative byte offset from it
+              const uint32_t boff = (uint32_t)(__mul24(offa[u], sgn2) + 2 * roi);
+              const uint16_t *rowp = dem + (omaj + sdir * (a + u)) * n + (omin - roi);
+              v = *(const uint16_t *)((const uint8_t *)rowp + boff);
+            } else {
+              const uint32_t prow = (uint32_t)(omaj + sdir * (a + u));
+              const uint32_t pcol = (uint32_t)(omin + __mul24(sgn, offa[u]));
+              v = dem[__umul24(prow, (uint32_t)n) + pcol];
+            }
+          }
+          pel[u] = v;
         }
         // (no instruction: makes the four reciprocals ready here, after the
         // DEM loads are issued and before the steps' conditional blocks, so
@@ -262,7 +282,7 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
         asm volatile("" ::"s"(rda[0]), "s"(rda[1]), "s"(rda[2]), "s"(rda[3]));
 #pragma unroll
         for (int u = 0; u < 4; u++)
-          step(!kChecked || a + u <= last, a + u, fa[u], rda[u], pel[u], offa[u]);
+          step(!kChecked || a + u <= last, a + u, (float)(a + u), rda[u], pel[u], offa[u]);
       };
       int a = 1;
       // steps every lane takes, then the rest, each lane up to its own last
