@@ -35,17 +35,19 @@ constexpr size_t kSplitBytes = 48 * 1024;  // budget of the two split bitsets
 // Where a workgroup keeps its observer's bitset while the rays run:
 //   kGlobal  straight in HBM (device atomics), for bitsets above kLdsWords
 //   kLds     one LDS bitset in the output layout
-//   kSplit   two LDS bitsets with an odd row stride ps: rows for the rays
-//            whose major axis is y, and a transposed one (a run of words per
-//            column) for the rays whose major axis is x.  At step a every
-//            x-major ray lies on one of two rows, so in the output layout
-//            their lanes would all OR into the same few words; transposed,
-//            each lane has a word of its own.  The transposed bitset is
-//            turned back, 64x64 bits per wavefront with ballots, when the
-//            output is written.
+//   kSplit   two LDS bitsets of 32-bit words, least significant bit first,
+//            with an odd row stride of ps dwords (so that neighbouring rows
+//            fall in different banks): rows for the rays whose major axis is
+//            y, and a transposed one (a run of words per column) for the rays
+//            whose major axis is x.  At step a every x-major ray lies on one
+//            of two rows, so in the output layout their lanes would all OR
+//            into the same few words; transposed, each lane has a word of its
+//            own.  Both are turned into the output's MSB-first 64-bit words
+//            (the transposed one 64x64 bits per wavefront) when the output is
+//            written.
 enum { kGlobal = 0, kLds = 1, kSplit = 2 };
 
-__device__ __forceinline__ int split_stride(int nwpr) { return nwpr | 1; }
+__device__ __host__ inline int split_stride(int nwpr) { return 2 * nwpr + 1; }
 
 // 64x64 bit transpose across a wavefront: lane l holds row l (bit c = column
 // c); returns lane l's column l as a row, i.e. result bit c = input bit l of
@@ -113,9 +115,10 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
   const int ps = split_stride(nwpr);
   uint64_t *out = sheds + (size_t)k * words;
   uint64_t *bits = kMode == kGlobal ? out : lds_dyn;
-  uint64_t *bits_t = lds_dyn + (size_t)nr * ps;  // kSplit: transposed bitset
+  uint32_t *b32 = (uint32_t *)lds_dyn;          // kSplit: row-major bitset
+  uint32_t *b32_t = b32 + (size_t)nr * ps;        // kSplit: transposed bitset
 
-  const int clear = kMode == kSplit ? 2 * nr * ps : words;
+  const int clear = kMode == kSplit ? nr * ps : words;  // 64-bit words
   for (int i = threadIdx.x; i < clear; i += nthreads) bits[i] = 0;
   if (kMode == kGlobal) __threadfence();
   __syncthreads();
@@ -126,9 +129,12 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
   const float foalt = (float)oalt;
   const int rr = roi * roi;
 
-  if (threadIdx.x == 0)  // observer sees itself, siting.cpp:279
-    atomicOr((unsigned long long *)&bits[roi * (kMode == kSplit ? ps : nwpr) + (roi >> 6)],
-             1ull << (63 - (roi & 63)));
+  if (threadIdx.x == 0) {  // observer sees itself, siting.cpp:279
+    if (kMode == kSplit)
+      atomicOr(&b32[roi * ps + (roi >> 5)], 1u << (roi & 31));
+    else
+      atomicOr((unsigned long long *)&bits[roi * nwpr + (roi >> 6)], 1ull << (63 - (roi & 63)));
+  }
 
   // Rays are dealt in passes of nthreads, the y-major ones first (the
   // interior cells of the rows dy = -roi and dy = roi), then the x-major ones
@@ -200,7 +206,7 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
       constexpr bool kUni = kCls != kAxisMixed;
       const bool ay = kCls == kAxisY ? true : kCls == kAxisX ? false : along_y;
       const int sdir = kUni ? __builtin_amdgcn_readfirstlane(dir) : dir;
-      const int sps8 = sgn * ps * 8;
+      const int sps4 = sgn * ps * 4;
       // siting.cpp:337-344 for one step.  The point slope is siting.cpp:340,
       // (float)(pel - oalt) / (float)a: with |pel - oalt| < 2^21 and
       // a < 2^14 the quotient is never closer than 2^-39 (relative) to a
@@ -221,18 +227,16 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
           const int c = kUni ? __builtin_amdgcn_readfirstlane(roi + sdir * a) : roi + sdir * a;
           if (kMode == kSplit && kUni) {
             // y-major: row off + roi, column c; x-major, transposed: column
-            // off + roi, row c.  One 32-bit half of the word takes the bit.
-            const int b = 63 - (c & 63);
-            const int wb = 8 * ((ay ? roi * ps : nr * ps + roi * ps) + (c >> 6)) + 4 * (b >> 5);
-            atomicOr((unsigned int *)((uint8_t *)bits + wb + __mul24(offa, sps8)), 1u << (b & 31));
+            // off + roi, row c
+            const int wb = 4 * ((ay ? roi * ps : nr * ps + roi * ps) + (c >> 5));
+            atomicOr((uint32_t *)((uint8_t *)b32 + wb + __mul24(offa, sps4)), 1u << (c & 31));
           } else {
             const int off = __mul24(sgn, offa);
             const int row = ay ? off + roi : c, col = ay ? c : off + roi;
             if (kMode == kSplit) {
-              const int w = ay ? (int)__umul24(row, ps) + (col >> 6)
-                               : nr * ps + (int)__umul24(col, ps) + (row >> 6);
-              const int bit = ay ? (col & 63) : (row & 63);
-              atomicOr((unsigned long long *)&bits[w], 1ull << (63 - bit));
+              const int w = ay ? (int)__umul24(row, ps) + (col >> 5)
+                               : nr * ps + (int)__umul24(col, ps) + (row >> 5);
+              atomicOr(&b32[w], 1u << ((ay ? col : row) & 31));
             } else {
               atomicOr((unsigned long long *)&bits[(int)__umul24(row, nwpr) + (col >> 6)],
                        1ull << (63 - (col & 63)));
@@ -303,20 +307,21 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
     for (int i = threadIdx.x; i < words; i += nthreads) out[i] = bits[i];
   } else if (kMode == kSplit) {
     // block (R, W): rows 64R..64R+63, output word W (columns 64W..64W+63).
-    // Lane l loads column 64W+l's transposed word for those rows (bit 63-r
-    // for row 64R+r).  Bit-reversed, that is row l of a matrix whose bit r
-    // is row 64R+r; transposed, lane l holds row 64R+l with bit c for
-    // column 64W+c, which bit-reversed is the output word (bit 63-c).
+    // Lane l loads column 64W+l's two transposed dwords for those rows (bit r
+    // for row 64R+r): row l of a matrix whose bit r is row 64R+r.
+    // Transposed, lane l holds row 64R+l with bit c for column 64W+c, which
+    // bit-reversed is the output word (bit 63-c), as is the row-major pair.
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave, nwaves = nthreads / kWave;
+    auto pair = [](const uint32_t *p) { return (uint64_t)p[0] | ((uint64_t)p[1] << 32); };
     for (int blk = wid; blk < nwpr * nwpr; blk += nwaves) {
       const int R = blk / nwpr, W = blk - R * nwpr;
       const int c = 64 * W + lane;
-      const uint64_t tw = c < nr ? bits_t[c * ps + R] : 0;
-      const uint64_t mine = __brevll(wave_transpose64(__brevll(tw), lane));
+      const uint64_t tw = c < nr ? pair(&b32_t[c * ps + 2 * R]) : 0;
+      const uint64_t mine = wave_transpose64(tw, lane);
       const int row = 64 * R + lane;
-      if (row < nr) out[row * nwpr + W] = bits[row * ps + W] | mine;
+      if (row < nr) out[row * nwpr + W] = __brevll(pair(&b32[row * ps + 2 * W]) | mine);
     }
   }
 }
@@ -365,7 +370,7 @@ siting_status launch_viewshed(const Geometry &g, const uint16_t *elevs, uint16_t
     const int t = std::atoi(e);
     if (t >= kWave && t <= kMaxThreads && t % kWave == 0) threads = t;
   }
-  const size_t split_bytes = 2 * sizeof(uint64_t) * g.d.nr * (size_t)(g.d.nwpr | 1);
+  const size_t split_bytes = 2 * sizeof(uint32_t) * g.d.nr * (size_t)split_stride(g.d.nwpr);
   const int mode = split_bytes <= kSplitBytes ? kSplit : words <= kLdsWords ? kLds : kGlobal;
   const size_t lds = mode == kSplit ? split_bytes
                      : mode == kLds ? sizeof(uint64_t) * words : 0;
