@@ -86,6 +86,10 @@ __device__ __forceinline__ float fmax_plain(float x, float y) {
   return r;
 }
 
+// Word 3 of a raw buffer resource on gfx9-family parts (32-bit data format,
+// no swizzle).
+constexpr int kBufferFlags = 0x00020000;
+
 // 0.5 - 2^-25, the largest float below one half: trunc(fl(x + kHalfDown))
 // rounds a non-negative float x below 2^15 half away from zero.
 constexpr float kHalfDown = 0.49999997f;
@@ -255,6 +259,9 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
       const uint16_t *dem = ay ? elevs_t : elevs;
       const int omaj = ay ? oy : ox, omin = ay ? ox : oy;
       const int sgn2 = 2 * sgn;
+      // the DEM as a buffer of 2 n^2 bytes (raw, no stride)
+      const __amdgpu_buffer_rsrc_t dres =
+          __builtin_amdgcn_make_buffer_rsrc((void *)dem, (short)0, 2 * n * n, kBufferFlags);
       auto group = [&](int a, auto checked_c) {
         constexpr bool kChecked = decltype(checked_c)::value;
         double rda[4];
@@ -267,11 +274,10 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
           int v = 0;
           if (!kChecked || a + u <= last) {
             if (kUni) {
-              // the row's cell omin - roi as a scalar base, the lane's cell
-              // as a non-negative byte offset from it
-              const uint32_t boff = (uint32_t)(__mul24(offa[u], sgn2) + 2 * roi);
-              const uint16_t *rowp = dem + (omaj + sdir * (a + u)) * n + (omin - roi);
-              v = *(const uint16_t *)((const uint8_t *)rowp + boff);
+              // a buffer load: the row's byte offset as the scalar offset,
+              // the lane's cell in the row as the vector one (both >= 0)
+              const int soff = 2 * (int)__umul24((uint32_t)(omaj + sdir * (a + u)), (uint32_t)n);
+              v = __builtin_amdgcn_raw_buffer_load_b16(dres, __mul24(offa[u], sgn2) + 2 * omin, soff, 0);
             } else {
               const uint32_t prow = (uint32_t)(omaj + sdir * (a + u));
               const uint32_t pcol = (uint32_t)(omin + __mul24(sgn, offa[u]));
