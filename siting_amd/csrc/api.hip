@@ -21,6 +21,13 @@ siting_status hip_fail(hipError_t e, const char *what, const char *file, int lin
   return SITING_EDEVICE;
 }
 
+// The SITE loop gave up at a grid barrier that did not complete (niter < 0).
+siting_status site_loop_failed() {
+  std::snprintf(g_last_error, sizeof g_last_error,
+                "SITE loop: a grid barrier timed out (workgroups not co-resident)");
+  return SITING_EDEVICE;
+}
+
 // siting.cpp:689-702, in float32 exactly as the reference evaluates it.
 siting_status derive(const siting_params *p, siting_derived *d) {
   if (!p || !d) return SITING_EINVAL;
@@ -376,6 +383,7 @@ siting_status siting_job_stats(siting_job *j, siting_stats *st) {
     st->sum_u = (int64_t)s.sum_u;
     const double n2 = (double)j->g.p.nrows * (double)j->g.p.nrows;
     st->coverage = 100.0 * (double)s.cumarea / n2;
+    if (s.niter < 0) return site_loop_failed();
   }
   return SITING_OK;
 }
@@ -583,6 +591,7 @@ siting_status siting_site_it(int nrows, int roi, int intervis, int nsheds,
     stats->sum_u = (int64_t)st.sum_u;
     stats->coverage = 100.0 * (double)st.cumarea / (double)n2;
   }
+  if (st.niter < 0) return site_loop_failed();
   return SITING_OK;
 }
 
@@ -739,7 +748,7 @@ void batch_drop_graph(siting_batch *b) {
 // took t ms on G workgroups is modelled to take t*sqrt(G/G') on G' (the
 // loop's rounds are latency-bound, so more groups help less than linearly).
 // Jobs, longest first, go to whichever slot would finish them first; the
-// mix of big slots (32, 48 or 64 groups, one to four of them) and slots of
+// mix of big slots (32 to 128 groups, one to four of them) and slots of
 // the base size with the smallest finish time is kept if it beats the
 // uniform slots by 10 %.  Each class then queues its jobs longest first.
 siting_status batch_replan(siting_batch *b) {
@@ -791,7 +800,7 @@ siting_status batch_replan(siting_batch *b) {
   const double m0 = schedule(0, b->slot, nullptr);
   int best_nb = 0, best_sb = 0;
   double best_m = m0;
-  for (int sb : {32, 48, 64}) {
+  for (int sb : {32, 48, 64, 96, 128}) {
     if (sb <= b->slot) continue;
     for (int nb = 1; nb <= 4; nb++) {
       if (total - nb * sb < b->slot) break;

@@ -214,18 +214,44 @@ __global__ __launch_bounds__(256) void site_first_pass(SiteArgs a) {
 // (An acquire load inside the spin would invalidate the cache on every poll,
 // and the polling groups would keep evicting the working set of the groups
 // still computing.)
-__device__ __forceinline__ void grid_sync(unsigned int *ctr, unsigned int &target,
+// Data the workgroups of a loop exchange (the published unit keys, the bits
+// each pick newly covered, the union cum) is only ever read and written with
+// device-coherent accesses: relaxed agent-scope atomic loads and stores
+// (sc1 on gfx950; cum takes device atomics), which need no cache
+// write-back or invalidation.  The grid barrier therefore only has to wait
+// for each wavefront's stores to complete before arriving, and uses no
+// agent-scope fence (on gfx950 those write back and invalidate the whole of
+// the XCD's L2, for every workgroup of every loop running).
+template <class T>
+__device__ __forceinline__ T ld_dev(const T *p) {
+  return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void st_dev(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Polls of the barrier counter before a workgroup gives the loop up (about
+// a second): a barrier that never completes ends the loop with niter = -1
+// instead of hanging the device.
+constexpr unsigned int kSpinLimit = 1u << 20;
+
+// Grid barrier over G resident workgroups; false if it timed out.
+__device__ __forceinline__ bool grid_sync(unsigned int *ctr, unsigned int &target,
                                           unsigned int G) {
+  __shared__ int timed_out;
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wavefront's stores are done
   __syncthreads();
   if (threadIdx.x == 0) {
     target += G;
-    __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    while ((int)(__hip_atomic_load(ctr, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT) - target) < 0)
+    __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned int spins = 0;
+    while ((int)(ld_dev(ctr) - target) < 0 && ++spins < kSpinLimit)
       __builtin_amdgcn_s_sleep(1);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    timed_out = spins >= kSpinLimit;
   }
   __syncthreads();
+  return !timed_out;
 }
 
 // Shifted word dk (0..nwpr) of row dr of the winner's shed, i.e. the bits of
@@ -287,7 +313,7 @@ __device__ __forceinline__ int lane_new_bits(const uint64_t *sh, int cx, int cy,
       const uint64_t *srow = sh + (size_t)row * nwpr;
       cur[u] = (ok && k < nwpr) ? srow[k] : 0;
       prv[u] = (ok && k > 0) ? srow[k - 1] : 0;
-      uint64_t c = ok ? a.cum[(size_t)(xo + row) * cw + (w0c + k)] : ~0ull;
+      uint64_t c = ok ? ld_dev(&a.cum[(size_t)(xo + row) * cw + (w0c + k)]) : ~0ull;
       const int dr = dro + row, dk = w0c + k - W.w0w;
       if (use_w && ok && dr >= 0 && dr < W.nr && dk >= 0 && dk <= W.nwpr)
         c |= W.word(dr, dk);
@@ -348,7 +374,7 @@ __device__ __forceinline__ int wave_lost_bits(const uint64_t *sh, int cx, int cy
         const bool ok = it < items;
         const int q = fdiv ? (int)(((float)it + 0.5f) * inv_nk) : it / nk;
         const int x = xlo + q, j = jlo + (it - q * nk);
-        nbv[u] = ok ? nbp[(size_t)(x - wx + roi) * dw + (j - w0p)] : 0;
+        nbv[u] = ok ? ld_dev(&nbp[(size_t)(x - wx + roi) * dw + (j - w0p)]) : 0;
         soff[u] = (x - cx + roi) * nwpr;
         kk[u] = j - w0c;
       }
@@ -550,8 +576,8 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     for (int e = tid; e < 2 * upw; e += kLoopThreads) {
       const int u = (e >> 1) * G + g;
       if (u < nunits) {
-        kn[2 * u + (e & 1)] = ukey[e];
-        pn[2 * u + (e & 1)] = upos[e];
+        st_dev(&kn[2 * u + (e & 1)], ukey[e]);
+        st_dev(&pn[2 * u + (e & 1)], upos[e]);
       }
     }
   };
@@ -560,12 +586,12 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   __syncthreads();
   publish(0);
   unsigned int bar_target = 0;
-  grid_sync(&a.st->barrier, bar_target, (unsigned)G);
+  const bool started = grid_sync(&a.st->barrier, bar_target, (unsigned)G);
 
-  int nused = 0, niter = 1, parity = 0;
+  int nused = 0, niter = started ? 1 : -1, parity = 0;
   long long cumarea = 0, my_u = 0;
 
-  for (;;) {
+  while (started) {
     // ---- pick the round's winners (identical in every workgroup).  Winners
     // are a prefix of the descending order of all published unit keys in
     // which no two are within 2*roi of each other: none of them is in
@@ -587,10 +613,10 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         for (int u0 = wid * kWave + lane; u0 < nunits; u0 += 2 * kPickWaves * kWave) {
           const int u1 = u0 + kPickWaves * kWave;
           const bool has1 = u1 < nunits;
-          const uint64_t a0 = kk[2 * u0], a1 = kk[2 * u0 + 1];
-          const uint32_t q0 = kp[2 * u0], q1 = kp[2 * u0 + 1];
-          const uint64_t b0 = has1 ? kk[2 * u1] : 0, b1 = has1 ? kk[2 * u1 + 1] : 0;
-          const uint32_t r0 = has1 ? kp[2 * u1] : 0, r1 = has1 ? kp[2 * u1 + 1] : 0;
+          const uint64_t a0 = ld_dev(&kk[2 * u0]), a1 = ld_dev(&kk[2 * u0 + 1]);
+          const uint32_t q0 = ld_dev(&kp[2 * u0]), q1 = ld_dev(&kp[2 * u0 + 1]);
+          const uint64_t b0 = has1 ? ld_dev(&kk[2 * u1]) : 0, b1 = has1 ? ld_dev(&kk[2 * u1 + 1]) : 0;
+          const uint32_t r0 = has1 ? ld_dev(&kp[2 * u1]) : 0, r1 = has1 ? ld_dev(&kp[2 * u1 + 1]) : 0;
           top3_push(h, a0, q0);
           top3_push(h, a1, q1 | kPickLast);
           top3_push(h, b0, r0);
@@ -746,7 +772,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         const int x = W.wx - roi + dr, j = W.w0w + dk;
         if (x < 0 || x >= n || j < 0 || j >= cw) continue;
         const uint64_t v = W.word(dr, dk);
-        if (v) a.cum[(size_t)x * cw + j] |= v;
+        if (v) atomicOr((unsigned long long *)&a.cum[(size_t)x * cw + j], (unsigned long long)v);
       }
     } else if (k > 0) {
       // every pick: OR its shifted shed into cum and keep the bits that were
@@ -774,9 +800,12 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
             nbv = v & ~old;
           }
         }
-        a.nb[t] = nbv;
+        st_dev(&a.nb[t], nbv);
       }
-      grid_sync(&a.st->barrier, bar_target, (unsigned)G);
+      if (!grid_sync(&a.st->barrier, bar_target, (unsigned)G)) {
+        niter = -1;
+        break;
+      }
     }
 
     // ---- update lists of picks 0..kc-1 among this workgroup's candidates.
@@ -852,7 +881,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           } else if (m) {
             int acc = m;
             if (!stop && a.intervis) {  // cpp:452-457 (one pick here)
-              uint64_t word = a.cum[(size_t)cx * cw + (cy >> 6)];
+              uint64_t word = ld_dev(&a.cum[(size_t)cx * cw + (cy >> 6)]);
               const int dr = cx - W.wx + roi, dk = (cy >> 6) - W.w0w;
               if (dr >= 0 && dr < nr && dk >= 0 && dk <= nwpr) word |= W.word(dr, dk);
               if (!((word >> (63 - (cy & 63))) & 1u)) acc = -m;
@@ -952,7 +981,10 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     for (int ul = tid; ul < upw; ul += kLoopThreads) dirty[ul] = 0;
     parity ^= 1;
     publish(parity);
-    grid_sync(&a.st->barrier, bar_target, (unsigned)G);
+    if (!grid_sync(&a.st->barrier, bar_target, (unsigned)G)) {
+      niter = -1;
+      break;
+    }
   }
 
   // sum_u: each thread counted its own list entries
@@ -1002,7 +1034,7 @@ __global__ __launch_bounds__(kLoopThreads) void site_loop_queue(
     if (g == 0 && threadIdx.x == 0)
       __hip_atomic_store(cell, atomicAdd(queue, 1u), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
-    grid_sync(slot_bar, target, (unsigned)slot);
+    if (!grid_sync(slot_bar, target, (unsigned)slot)) break;
     if (threadIdx.x == 0)
       job = (int)__hip_atomic_load(cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
