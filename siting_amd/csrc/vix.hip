@@ -30,6 +30,12 @@ constexpr int kThreads = kTile * kTile;
 template <int kHalo>
 constexpr int win_edge() { return kTile + 2 * kHalo; }
 
+// 0.5 - 2^-25, the largest float below one half: trunc(fl(x + kHalfDown))
+// rounds a non-negative float x below 2^15 half away from zero.
+constexpr float kHalfDown = 0.49999997f;
+// The reciprocal table (roi + 1 floats) is copied to LDS up to this roi.
+constexpr int kRecipLdsMax = 2048;
+
 // siting.cpp:157,160, evaluated in float in the reference's order:
 // ((float)(2roi) + 0.99999f) * (float)r / 2^31, truncated.
 __device__ __forceinline__ int target_offset(float span, uint32_t r) {
@@ -38,7 +44,7 @@ __device__ __forceinline__ int target_offset(float span, uint32_t r) {
   return (int)q;
 }
 
-template <int kGroup, int kHalo>
+template <int kHalo, bool kRecipLds>
 __global__ __launch_bounds__(kThreads) void vix_kernel(
     const uint16_t *__restrict__ elevs, uint8_t *__restrict__ vix,
     const float *__restrict__ recip, const uint32_t *__restrict__ jump,
@@ -49,6 +55,7 @@ __global__ __launch_bounds__(kThreads) void vix_kernel(
   int *count = (int *)(lds + 4 * ntests);  // kThreads
   constexpr int kWin = win_edge<kHalo>();
   uint16_t *win = (uint16_t *)(count + kThreads);  // kWin x kWin
+  float *rl = (float *)(count + kThreads + (kWin * kWin + 1) / 2);  // roi + 1 (kRecipLds)
 
   const int b = blockIdx.x;
   const int tile = (b & 7) * tiles_per_xcd + (b >> 3);
@@ -57,6 +64,9 @@ __global__ __launch_bounds__(kThreads) void vix_kernel(
   const int ty0 = (tile % tiles_side) * kTile;
 
   for (int i = threadIdx.x; i < 4 * ntests; i += kThreads) jl[i] = jump[i];
+  if (kRecipLds)
+    for (int i = threadIdx.x; i <= roi; i += kThreads) rl[i] = recip[i];
+  const float *rtab = kRecipLds ? rl : recip;
   count[threadIdx.x] = 0;
   const int wx0 = tx0 - kHalo, wy0 = ty0 - kHalo;
   for (int i = threadIdx.x; i < kWin * kWin; i += kThreads) {
@@ -101,48 +111,58 @@ __global__ __launch_bounds__(kThreads) void vix_kernel(
           const int major = alx ? dx : dy;
           const int minor = alx ? dy : dx;
           const int len = alx ? adx : ady;
-          float rec = recip[len];
+          float rec = rtab[len];
           if (major < 0) rec = -rec;  // (float)(1.0/-d) == -(float)(1.0/d)
           const float slope = (float)minor * rec;
           const int dir = major > 0 ? 1 : -1;
-          // The reference samples id = dir*1, dir*2, dir*4, ... while |id| <
-          // len and stops at the first sample above the sight line, so the
-          // target is visible iff no sample is above it: the samples can be
-          // read in any order.  kGroup at a time, the first group in flight
-          // together with the observer and target elevations.
-          const int oz_raw = (int)win[(px - wx0) * kWin + (py - wy0)];
-          const int tz_raw = (int)elevs[tx * n + ty];
-          float zz[kGroup];
-          auto load_group = [&](int s0) {
-#pragma unroll
-            for (int u = 0; u < kGroup; u++) {
-              const int sv = s0 << u;
-              const int id = dir * sv;
-              const int off = round_half_away(slope * (float)id);
-              const int sx = alx ? px + id : px + off;
-              const int sy = alx ? py + off : py + id;
-              zz[u] = sv >= len ? 0.0f
-                      : sv <= kHalo ? (float)win[(sx - wx0) * kWin + (sy - wy0)]
-                                    : (float)elevs[sx * n + sy];
-            }
-          };
-          load_group(1);
-          const int oz = oz_raw + oht;
-          const int tz = tz_raw + tht;
+          const int oz = (int)win[(px - wx0) * kWin + (py - wy0)] + oht;
+          const int tz = (int)elevs[tx * n + ty] + tht;
           const float zs = (float)(tz - oz) * rec;
           const float foz = (float)oz;
+          // The reference samples id = dir*1, dir*2, dir*4, ... while |id| <
+          // len and stops at the first sample above the sight line, so the
+          // target is visible iff no sample is above it.  With |id| a power
+          // of two, slope*id and id*zs are exact scalings of slope and zs:
+          // both double from one sample to the next, exactly.  The minor
+          // offset rounds slope*id half away from zero, i.e. sign *
+          // trunc(fl(|slope*id| + (0.5 - 2^-25))).  Samples up to kHalo
+          // steps away are read from the LDS window.
+          const bool neg = (slope < 0.0f) != (dir < 0);  // sign of slope*id
+          float xs = fabsf(slope);                        // |slope*id| at |id| = 1
+          float ts = dir > 0 ? zs : -zs;                  // id*zs at |id| = 1
+          // cell of sample (id, off) in the window and in the DEM: base +
+          // |id| * (major step) + off * (minor step) (24-bit products: the
+          // window edge, |id|, |off| and n are all below 2^15)
+          const int wmaj = alx ? dir * kWin : dir, wmin = alx ? 1 : kWin;
+          const int gmaj = alx ? dir * n : dir, gmin = alx ? 1 : n;
+          const int wbase = (px - wx0) * kWin + (py - wy0);
+          const int gbase = px * n + py;
           vis = true;
-          for (int s0 = 1;;) {
-#pragma unroll
-            for (int u = 0; u < kGroup; u++) {
-              const int sv = s0 << u;
-              const float fid = (float)(dir * sv);
-              const float lim = foz + fid * zs;
-              if (sv < len && zz[u] > lim) vis = false;
+          int sv = 1;
+          for (; sv < len && sv <= kHalo; sv <<= 1) {
+            const int oa = (int)__fadd_rn(xs, kHalfDown);
+            const int off = neg ? -oa : oa;
+            const float z = (float)win[wbase + __mul24(sv, wmaj) + __mul24(off, wmin)];
+            if (z > __fadd_rn(foz, ts)) {
+              vis = false;
+              break;
             }
-            s0 <<= kGroup;
-            if (!vis || s0 >= len) break;
-            load_group(s0);
+            xs = __fadd_rn(xs, xs);
+            ts = __fadd_rn(ts, ts);
+          }
+          if (vis) {
+            for (; sv < len; sv <<= 1) {
+              const int oa = (int)__fadd_rn(xs, kHalfDown);
+              const int off = neg ? -oa : oa;
+              const uint32_t cell = (uint32_t)(gbase + __mul24(sv, gmaj) + __mul24(off, gmin));
+              const float z = (float)elevs[cell];
+              if (z > __fadd_rn(foz, ts)) {
+                vis = false;
+                break;
+              }
+              xs = __fadd_rn(xs, xs);
+              ts = __fadd_rn(ts, ts);
+            }
           }
         }
       }
@@ -203,22 +223,27 @@ siting_status launch_vix(const Geometry &g, const uint16_t *elevs, uint8_t *vix,
   const int side = (n + kTile - 1) / kTile;
   const int ntiles = side * side;
   const int per_xcd = (ntiles + 7) / 8;
-  // samples read per dependent step and LDS halo (tuning overrides
-  // SITING_VIX_GROUP, SITING_VIX_HALO)
-  int group = 1, halo = 16;
-  if (const char *e = std::getenv("SITING_VIX_GROUP")) group = std::atoi(e);
+  // LDS halo (tuning override SITING_VIX_HALO)
+  int halo = 16;
   if (const char *e = std::getenv("SITING_VIX_HALO")) halo = std::atoi(e);
+  const bool rlds = g.p.roi <= kRecipLdsMax;
   const size_t base = sizeof(uint32_t) * (4 * (size_t)g.p.ntests + kThreads);
-#define SITING_VIX(G, H)                                                             \
-  hipLaunchKernelGGL((vix_kernel<G, H>), dim3(8 * per_xcd), dim3(kThreads),             \
-                     base + sizeof(uint16_t) * win_edge<H>() * win_edge<H>(), s, elevs, \
-                     vix, t.recip, t.jump, n, g.p.roi, g.p.oht, g.p.tht, g.p.ntests,   \
-                     side, per_xcd)
-  if (group == 4) SITING_VIX(4, 16);
-  else if (group == 2) SITING_VIX(2, 16);
-  else if (halo == 8) SITING_VIX(1, 8);
-  else if (halo == 32) SITING_VIX(1, 32);
-  else SITING_VIX(1, 16);
+#define SITING_VIX(H, L)                                                                \
+  hipLaunchKernelGGL((vix_kernel<H, L>), dim3(8 * per_xcd), dim3(kThreads),                \
+                     base + sizeof(uint32_t) * ((win_edge<H>() * win_edge<H>() + 1) / 2) + \
+                         (L ? sizeof(float) * (g.p.roi + 1) : 0),                          \
+                     s, elevs, vix, t.recip, t.jump, n, g.p.roi, g.p.oht, g.p.tht,         \
+                     g.p.ntests, side, per_xcd)
+  if (halo == 8) {
+    if (rlds) SITING_VIX(8, true);
+    else SITING_VIX(8, false);
+  } else if (halo == 32) {
+    if (rlds) SITING_VIX(32, true);
+    else SITING_VIX(32, false);
+  } else {
+    if (rlds) SITING_VIX(16, true);
+    else SITING_VIX(16, false);
+  }
 #undef SITING_VIX
   SITING_HIP(hipGetLastError());
   return SITING_OK;
