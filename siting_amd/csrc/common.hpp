@@ -53,16 +53,10 @@ __host__ __device__ inline void col_split(int c, int &word, int &bit) {
   bit = b;
 }
 
-// Round half away from zero, as libm round() (siting.cpp:95,330).  x - t is
-// exact for |x| < 2^23, so the comparison against 0.5 is exact.
-__device__ __forceinline__ int round_half_away(float x) {
-  float t = __builtin_truncf(x);
-  float f = x - t;
-  int r = (int)t;
-  r += (f >= 0.5f) ? 1 : 0;
-  r -= (f <= -0.5f) ? 1 : 0;
-  return r;
-}
+// 0.5 - 2^-25, the largest float below one half.  libm round() of a
+// non-negative float x below 2^15 (siting.cpp:95,330 round half away from
+// zero) is trunc(fl(x + kHalfDown)): checked for every such float.
+constexpr float kHalfDown = 0.49999997f;
 
 // ---------------------------------------------------------------------------
 // wave / block reductions

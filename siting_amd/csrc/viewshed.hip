@@ -90,9 +90,6 @@ __device__ __forceinline__ float fmax_plain(float x, float y) {
 // no swizzle).
 constexpr int kBufferFlags = 0x00020000;
 
-// 0.5 - 2^-25, the largest float below one half: trunc(fl(x + kHalfDown))
-// rounds a non-negative float x below 2^15 half away from zero.
-constexpr float kHalfDown = 0.49999997f;
 
 __device__ __forceinline__ int wave_max_i32(int v) {
 #pragma unroll

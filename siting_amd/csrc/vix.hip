@@ -30,9 +30,6 @@ constexpr int kThreads = kTile * kTile;
 template <int kHalo>
 constexpr int win_edge() { return kTile + 2 * kHalo; }
 
-// 0.5 - 2^-25, the largest float below one half: trunc(fl(x + kHalfDown))
-// rounds a non-negative float x below 2^15 half away from zero.
-constexpr float kHalfDown = 0.49999997f;
 // The reciprocal table (roi + 1 floats) is copied to LDS up to this roi.
 constexpr int kRecipLdsMax = 2048;
 

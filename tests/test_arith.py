@@ -24,8 +24,9 @@ def test_division_by_double_reciprocal_matches_fp32_division():
 
 
 def test_round_half_away_by_trunc():
-    """round_half_away(x) = trunc(x) + (frac >= .5) - (frac <= -.5) equals
-    libm round() on float32 values, exact halves included."""
+    """The tests model the reference's round() (siting.cpp:95,330) as
+    trunc(x) + (frac >= .5) - (frac <= -.5); pinned here against libm
+    round() on float32 values, exact halves included."""
     rng = np.random.default_rng(2)
     x = np.concatenate([rng.uniform(-20000, 20000, 200000).astype(np.float32),
                         (np.arange(-4000, 4001) / 2).astype(np.float32),
@@ -131,8 +132,9 @@ def test_ray_set_is_the_box_perimeter():
 
 
 def test_half_away_rounding_by_biased_truncation():
-    """VIEWSHED rounds a ray's minor offset fl(i * slope) half away from zero
-    (siting.cpp:330) as sign * trunc(fl(fl(a * |slope|) + (0.5 - 2^-25))).
+    """VIEWSHED and VIX round a ray's minor offset fl(i * slope) half away
+    from zero (siting.cpp:95,330) as sign * trunc(fl(|fl(i * slope)| +
+    (0.5 - 2^-25))), with |fl(i * slope)| = fl(|i| * |slope|).
     The magnitude identity floor(x + 0.5) == trunc(fl(x + 0.49999997f)) is
     checked here for every float32 x in [0, 2^15) (offsets never exceed
     roi < 2^14), and the whole offset against the reference formula for
