@@ -162,6 +162,7 @@ struct SiteArgs {
                   // of the round's largest become stale bounds instead of
                   // being decremented (0: never)
   long long stop_area;  // least cumulative area c with 100.0 * c / (n * n) > 95
+  unsigned int spin_limit;  // barrier polls before the loop gives up
 };
 
 // Device scratch SITE needs beyond SiteBuffers' arrays, and the size of the

@@ -236,9 +236,10 @@ __device__ __forceinline__ void st_dev(T *p, T v) {
 // instead of hanging the device.
 constexpr unsigned int kSpinLimit = 1u << 20;
 
-// Grid barrier over G resident workgroups; false if it timed out.
+// Grid barrier over G resident workgroups; false if it timed out (a limit
+// of 0 gives up at once).
 __device__ __forceinline__ bool grid_sync(unsigned int *ctr, unsigned int &target,
-                                          unsigned int G) {
+                                          unsigned int G, unsigned int limit = kSpinLimit) {
   __shared__ int timed_out;
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wavefront's stores are done
   __syncthreads();
@@ -246,9 +247,9 @@ __device__ __forceinline__ bool grid_sync(unsigned int *ctr, unsigned int &targe
     target += G;
     __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     unsigned int spins = 0;
-    while ((int)(ld_dev(ctr) - target) < 0 && ++spins < kSpinLimit)
+    while (limit != 0 && (int)(ld_dev(ctr) - target) < 0 && ++spins < limit)
       __builtin_amdgcn_s_sleep(1);
-    timed_out = spins >= kSpinLimit;
+    timed_out = spins >= limit;
   }
   __syncthreads();
   return !timed_out;
@@ -586,7 +587,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   __syncthreads();
   publish(0);
   unsigned int bar_target = 0;
-  const bool started = grid_sync(&a.st->barrier, bar_target, (unsigned)G);
+  const bool started = grid_sync(&a.st->barrier, bar_target, (unsigned)G, a.spin_limit);
 
   int nused = 0, niter = started ? 1 : -1, parity = 0;
   long long cumarea = 0, my_u = 0;
@@ -802,7 +803,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         }
         st_dev(&a.nb[t], nbv);
       }
-      if (!grid_sync(&a.st->barrier, bar_target, (unsigned)G)) {
+      if (!grid_sync(&a.st->barrier, bar_target, (unsigned)G, a.spin_limit)) {
         niter = -1;
         break;
       }
@@ -981,7 +982,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     for (int ul = tid; ul < upw; ul += kLoopThreads) dirty[ul] = 0;
     parity ^= 1;
     publish(parity);
-    if (!grid_sync(&a.st->barrier, bar_target, (unsigned)G)) {
+    if (!grid_sync(&a.st->barrier, bar_target, (unsigned)G, a.spin_limit)) {
       niter = -1;
       break;
     }
@@ -1272,6 +1273,10 @@ siting_status site_make_args(const Geometry &g, const int32_t *obs,
   a.o_seen = p.o_seen;
   a.tau8 = tau8();
   a.stop_area = stop_area(n);
+  // (a test forces the give-up path with a tiny limit, every kernel still
+  // exiting normally)
+  const char *sl = std::getenv("SITING_SITE_SPIN_LIMIT");
+  a.spin_limit = sl && *sl && std::atoi(sl) >= 0 ? (unsigned)std::atoi(sl) : kSpinLimit;
   *out = a;
   if (lds) *lds = p.lds;
   return SITING_OK;
