@@ -71,6 +71,9 @@ constexpr int kLoopWaves = kLoopThreads / kWave;
 constexpr int kMaxList = kLoopThreads;  // list windows: one item per thread
 constexpr size_t kLdsBudget = 132 * 1024;  // dynamic LDS of the loop kernel
 constexpr int kMaxGroups = 128;
+// Candidates one loop workgroup keeps in LDS comfortably (the budget holds
+// about 9000 at 14 bytes each).
+constexpr int kFitPerGroup = 8000;
 constexpr int kMaxBatch = kSiteMaxBatch;
 // Wavefronts that gather the published unit keys for the pick walk, and the
 // position bit (bit 15: y < 32768) that flags a key whose successor in its
@@ -1120,9 +1123,11 @@ Plan make_plan(int n, int roi, int nsheds, int groups_cap, int fixed_groups = 0)
   p.Q = choose_q(n, roi, nsheds);
   if (p.Q > p.ncr) p.Q = p.ncr;
   p.nunits = p.ncr * p.Q;
-  // units in a pick's 4*roi window, times workgroups per such unit
+  // units in a pick's 4*roi window, times workgroups per such unit, and at
+  // least enough workgroups that each one's candidates fit its LDS lists
   const int touched = ((4 * roi) / p.cs + 2) * p.Q;
-  int G = fixed_groups > 0 ? fixed_groups : touched;
+  const int fit = (nsheds + kFitPerGroup - 1) / kFitPerGroup;
+  int G = fixed_groups > 0 ? fixed_groups : (touched > fit ? touched : fit);
   G = G < p.nunits ? G : p.nunits;
   G = G < groups_cap ? G : groups_cap;
   G = G < kMaxGroups ? G : kMaxGroups;
@@ -1315,6 +1320,8 @@ siting_status launch_site(const Geometry &g, const int32_t *obs,
   // resource that a second process on the same GPU cannot get.
   int cap = cus / 2 > 0 ? cus / 2 : 1;
   if (groups_cap > 0 && groups_cap < cap) cap = groups_cap;
+  if (const char *e = std::getenv("SITING_SITE_GROUPS"))  // tuning override
+    if (std::atoi(e) > 0 && std::atoi(e) < cap) cap = std::atoi(e);
   const Plan p = make_plan(g.p.nrows, g.p.roi, g.d.nwanted, cap);
   SiteArgs a;
   size_t lds = 0;
