@@ -158,9 +158,10 @@ struct SiteArgs {
   int o_delta, o_ukey, o_upos, o_dirty, o_off, o_c, o_pos, o_t, o_used, o_seen;
   int off_lds;    // unit cell offsets kept in LDS (else read from bstart)
   int delta_lds;  // the winner's shifted shed staged in LDS (else recomputed)
-  int tau8;       // intervis 0: update candidates whose area is below tau8/8
-                  // of the round's largest become stale bounds instead of
-                  // being decremented (0: never)
+  int tau_shift;  // intervis 0: update candidates whose area is below
+                  // head - head / 2^tau_shift (head: the round's largest
+                  // key's area) become stale bounds instead of being
+                  // decremented (0: never)
   long long stop_area;  // least cumulative area c with 100.0 * c / (n * n) > 95
   unsigned int spin_limit;  // barrier polls before the loop gives up
 };

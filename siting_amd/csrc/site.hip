@@ -511,7 +511,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   const size_t S = (size_t)nr * nwpr;
   const int dw = nwpr + 1;
   const int kmax = a.intervis ? 1 : kMaxBatch;
-  const bool hybrid = !a.intervis && a.tau8 > 0;
+  const bool hybrid = !a.intervis && a.tau_shift > 0;
 
   uint64_t *delta = (uint64_t *)(smem + a.o_delta);
   uint64_t *ukey = (uint64_t *)(smem + a.o_ukey);   // [upw][2]
@@ -694,7 +694,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     if (k == 0 && !eval_only) break;  // cpp:589-592
     // update candidates below tau become stale; stale ones reaching tau are
     // re-evaluated (areas only shrink, so a bound never falls below the area)
-    const int tau = hybrid ? (int)(((long long)bw_head * a.tau8) >> 3) : 0;
+    const int tau = hybrid ? bw_head - (bw_head >> a.tau_shift) : 0;
     if (nused + k > a.nsheds) {  // cannot happen: every pick retires a candidate
       niter = -1;
       break;
@@ -1080,13 +1080,13 @@ int choose_q(int n, int roi, int nsheds) {
   return list >= 3000.0 ? 3 : 2;
 }
 
-// Update candidates whose area is below tau8/8 of the round's largest are
-// not decremented but kept as stale bounds (SITING_SITE_TAU8 overrides; 0
-// turns it off).
-int tau8() {
-  const char *e = std::getenv("SITING_SITE_TAU8");
-  const int t = e && *e ? std::atoi(e) : 7;
-  return t < 0 ? 0 : (t > 8 ? 8 : t);
+// Update candidates whose area is below tau = head - head / 2^s (head: the
+// round's largest key's area) are not decremented but kept as stale bounds
+// (SITING_SITE_TAU_SHIFT sets s; 0 turns it off).
+int tau_shift() {
+  const char *e = std::getenv("SITING_SITE_TAU_SHIFT");
+  const int t = e && *e ? std::atoi(e) : 5;
+  return t < 0 ? 0 : (t > 20 ? 20 : t);
 }
 
 // Cell edge: about 30 cell rows across the 4*roi update window, and at most
@@ -1276,7 +1276,7 @@ siting_status site_make_args(const Geometry &g, const int32_t *obs,
   a.o_t = p.o_t;
   a.o_used = p.o_used;
   a.o_seen = p.o_seen;
-  a.tau8 = tau8();
+  a.tau_shift = tau_shift();
   a.stop_area = stop_area(n);
   // (a test forces the give-up path with a tiny limit, every kernel still
   // exiting normally)
