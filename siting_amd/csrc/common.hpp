@@ -132,7 +132,7 @@ struct SiteBuffers {
 };
 
 // Everything one job's SITE kernels need (device pointers, geometry, plan).
-constexpr int kSiteMaxBatch = 16;  // picks per round (intervis 0)
+constexpr int kSiteMaxBatch = 32;  // picks per round (intervis 0)
 struct SiteArgs {
   const int32_t *obs;
   const uint64_t *sheds;

@@ -79,7 +79,7 @@ constexpr int kMaxBatch = kSiteMaxBatch;
 // position bit (bit 15: y < 32768) that flags a key whose successor in its
 // unit or lane list is unknown.
 constexpr int kPickWaves = 4;
-static_assert(kMaxBatch <= kLoopWaves, "one wavefront per pick marks it used");
+static_assert(kMaxBatch <= kWave, "lane j of the walk holds pick j");
 constexpr uint32_t kPickLast = 0x8000u;
 
 
@@ -704,33 +704,36 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     bool stop = false;
     int kc = k;  // picks whose update lists the sequential loop evaluates
     {
-      uint64_t key[kMaxBatch];
+      // every wavefront: lane i holds pick i's area, an inclusive prefix sum
+      // gives the cumulative area after each pick, and the first pick that
+      // reaches the stop area ends the round (cpp:601-604)
+      long long c = lane < k ? (long long)(bw_key[lane] >> 32) : 0;
 #pragma unroll
-      for (int i = 0; i < kMaxBatch; i++) key[i] = i < k ? bw_key[i] : 0;
-      const int nused0 = nused;
-#pragma unroll
-      for (int i = 0; i < kMaxBatch; i++) {
-        if (i >= k) break;
-        nused++;
-        cumarea += (long long)(key[i] >> 32);
-        if (cumarea >= a.stop_area) {  // cpp:601-604
-          stop = true;
-          k = i + 1;
-          kc = i;
-          break;
-        }
-        niter++;
+      for (int o = 1; o < kWave; o <<= 1) {
+        const long long up = __shfl_up(c, o, kWave);
+        if (lane >= o) c += up;
       }
+      const uint64_t hits = __ballot(lane < k && cumarea + c >= a.stop_area);
+      const int nused0 = nused;
+      if (hits) {
+        const int i = (int)__builtin_ctzll(hits);
+        stop = true;
+        k = i + 1;
+        kc = i;
+      }
+      nused += k;
+      niter += kc;
+      cumarea += __shfl(c, k - 1, kWave);
       if (g == 0 && tid < k) {
         const int w = (int)(~(uint32_t)bw_key[tid]);
         a.used[w] = 1;
         a.order[nused0 + tid] = w;
       }
     }
-    // owners mark the picks used: wavefront i takes pick i
-    if (wid < k) {
-      const int w = (int)(~(uint32_t)bw_key[wid]);
-      const int wx = (int)(bw_pos[wid] >> 16), wy = (int)(bw_pos[wid] & 0xffffu);
+    // owners mark the picks used: wavefront i takes picks i, i + 16, ...
+    for (int pi = wid; pi < k; pi += kLoopWaves) {
+      const int w = (int)(~(uint32_t)bw_key[pi]);
+      const int wx = (int)(bw_pos[pi] >> 16), wy = (int)(bw_pos[pi] & 0xffffu);
       const int w_r = wx / a.cs, w_cc = wy / a.cs;
       const int w_u = w_r * Q + w_cc % Q;
       if (w_u % G == g) {
