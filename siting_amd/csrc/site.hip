@@ -239,16 +239,24 @@ __device__ __forceinline__ void st_dev(T *p, T v) {
 // instead of hanging the device.
 constexpr unsigned int kSpinLimit = 1u << 20;
 
-// Grid barrier over G resident workgroups; false if it timed out (a limit
-// of 0 gives up at once).
-__device__ __forceinline__ bool grid_sync(unsigned int *ctr, unsigned int &target,
-                                          unsigned int G, unsigned int limit = kSpinLimit) {
-  __shared__ int timed_out;
+// Grid barrier over G resident workgroups, in two halves so that work that
+// needs nothing from the other workgroups can run in between: arrive once
+// this workgroup's stores are done, wait for every arrival (false if the
+// wait timed out; a limit of 0 gives up at once).  target is thread 0's.
+__device__ __forceinline__ void grid_arrive(unsigned int *ctr, unsigned int &target,
+                                            unsigned int G) {
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wavefront's stores are done
   __syncthreads();
   if (threadIdx.x == 0) {
     target += G;
     __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__device__ __forceinline__ bool grid_wait(unsigned int *ctr, unsigned int target,
+                                          unsigned int limit) {
+  __shared__ int timed_out;
+  if (threadIdx.x == 0) {
     unsigned int spins = 0;
     while (limit != 0 && (int)(ld_dev(ctr) - target) < 0 && ++spins < limit)
       __builtin_amdgcn_s_sleep(1);
@@ -256,6 +264,12 @@ __device__ __forceinline__ bool grid_sync(unsigned int *ctr, unsigned int &targe
   }
   __syncthreads();
   return !timed_out;
+}
+
+__device__ __forceinline__ bool grid_sync(unsigned int *ctr, unsigned int &target,
+                                          unsigned int G, unsigned int limit = kSpinLimit) {
+  grid_arrive(ctr, target, G);
+  return grid_wait(ctr, target, limit);
 }
 
 // Shifted word dk (0..nwpr) of row dr of the winner's shed, i.e. the bits of
@@ -702,6 +716,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
 
     // ---- bookkeeping in pick order; the 95% rule may cut the round short
     bool stop = false;
+    bool union_pending = false;  // arrived at the union barrier, not waited
     int kc = k;  // picks whose update lists the sequential loop evaluates
     {
       // every wavefront: lane i holds pick i's area, an inclusive prefix sum
@@ -809,11 +824,18 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         }
         st_dev(&a.nb[t], nbv);
       }
-      if (!grid_sync(&a.st->barrier, bar_target, (unsigned)G, a.spin_limit)) {
-        niter = -1;
-        break;
-      }
+      // the update lists are built before the wait; nb and cum are read
+      // after it
+      grid_arrive(&a.st->barrier, bar_target, (unsigned)G);
+      union_pending = true;
     }
+    bool failed = false;
+    auto union_wait = [&]() {
+      if (union_pending) {
+        union_pending = false;
+        failed = !grid_wait(&a.st->barrier, bar_target, a.spin_limit);
+      }
+    };
 
     // ---- update lists of picks 0..kc-1 among this workgroup's candidates.
     // Ranges: for each pick, the cells of owned units inside the pick's
@@ -907,7 +929,8 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         my_u += v < 0 ? -v : v;
       }
       __syncthreads();  // counts read before the areas overwrite them
-      if (!stop && lcnt > 0) {
+      if (!stop && lcnt > 0) union_wait();
+      if (!stop && lcnt > 0 && !failed) {
         // one wavefront per update candidate
         for (int e = wid; e < lcnt; e += kLoopWaves) {
           if (list_acc[e] < 0) continue;
@@ -934,6 +957,11 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     }
     for (int i = tid; i < (cnt + 31) / 32; i += kLoopThreads) seen[i] = 0;
     if (stop) break;
+    union_wait();  // before the re-evaluation reads cum
+    if (failed) {
+      niter = -1;
+      break;
+    }
     __syncthreads();
     if (hybrid) {
       // re-evaluate the stale candidates whose bound reaches tau against cum
