@@ -624,6 +624,10 @@ struct siting_batch {
 namespace {
 
 // Host form of the work-queue plan (layout in common.hpp).
+// Slot classes of a work-queue plan: a slot for the longest job, big slots
+// and base slots.
+constexpr int kMaxClasses = 3;
+
 struct QueuePlan {
   std::vector<int> start, size, cls, off, order;
   int total() const {
@@ -748,7 +752,8 @@ void batch_drop_graph(siting_batch *b) {
 // took t ms on G workgroups is modelled to take t*sqrt(G/G') on G' (the
 // loop's rounds are latency-bound, so more groups help less than linearly).
 // Jobs, longest first, go to whichever slot would finish them first; the
-// mix of big slots (32 to 128 groups, one to four of them) and slots of
+// mix of an optional slot of 64-128 groups for the longest job, big slots
+// (32 to 128 groups, up to four of them) and slots of
 // the base size with the smallest finish time is kept if it beats the
 // uniform slots by 10 %.  Each class then queues its jobs longest first.
 siting_status batch_replan(siting_batch *b) {
@@ -771,15 +776,21 @@ siting_status batch_replan(siting_batch *b) {
   std::stable_sort(by_time.begin(), by_time.end(),
                    [&](int x, int y) { return ms[(size_t)x] > ms[(size_t)y]; });
   const int total = b->total_groups;
-  // list schedule over nb slots of size sb and slots of the base size;
-  // returns the makespan, fills each job's class (0 big, 1 base)
-  auto schedule = [&](int nb, int sb, std::vector<int> *cls) {
-    std::vector<double> fin;
-    std::vector<int> size;
-    for (int k = 0; k < nb; k++) size.push_back(sb);
-    for (int k = 0; k < (total - nb * sb) / b->slot; k++) size.push_back(b->slot);
-    fin.assign(size.size(), 0.0);
-    if (cls) cls->assign((size_t)njobs, 1);
+  // slot sizes of a candidate plan: an optional slot of s1 groups for the
+  // longest job, nb slots of sb groups, and base slots for the rest
+  auto sizes_of = [&](int s1, int nb, int sb) {
+    std::vector<int> sz;
+    if (s1 > 0) sz.push_back(s1);
+    for (int k = 0; k < nb; k++) sz.push_back(sb);
+    const int used = (s1 > 0 ? s1 : 0) + nb * sb;
+    for (int k = 0; k < (total - used) / b->slot; k++) sz.push_back(b->slot);
+    return sz;
+  };
+  // list schedule over slots of the given sizes; returns the makespan and
+  // fills each job's slot size
+  auto schedule = [&](const std::vector<int> &size, std::vector<int> *slot_of) {
+    std::vector<double> fin(size.size(), 0.0);
+    if (slot_of) slot_of->assign((size_t)njobs, b->slot);
     for (int i : by_time) {
       size_t best = 0;
       double bf = 1e300;
@@ -791,51 +802,56 @@ siting_status batch_replan(siting_batch *b) {
         }
       }
       fin[best] = bf;
-      if (cls) (*cls)[(size_t)i] = size[best] == b->slot && (int)best >= nb ? 1 : 0;
+      if (slot_of) (*slot_of)[(size_t)i] = size[best];
     }
     double m = 0;
     for (double f : fin) m = f > m ? f : m;
     return m;
   };
-  const double m0 = schedule(0, b->slot, nullptr);
-  int best_nb = 0, best_sb = 0;
+  const double m0 = schedule(sizes_of(0, 0, b->slot), nullptr);
+  std::vector<int> best_sizes;
   double best_m = m0;
-  for (int sb : {32, 48, 64, 96, 128}) {
-    if (sb <= b->slot) continue;
-    for (int nb = 1; nb <= 4; nb++) {
-      if (total - nb * sb < b->slot) break;
-      const double m = schedule(nb, sb, nullptr);
-      if (m < best_m) {
-        best_m = m;
-        best_nb = nb;
-        best_sb = sb;
+  for (int s1 : {0, 64, 96, 128})
+    for (int sb : {32, 48, 64, 96, 128})
+      for (int nb = 0; nb <= 4; nb++) {
+        if (sb <= b->slot || (s1 > 0 ? s1 : 0) + nb * sb > total - b->slot) continue;
+        if (s1 == 0 && nb == 0) continue;
+        const std::vector<int> sz = sizes_of(s1, nb, sb);
+        const double m = schedule(sz, nullptr);
+        if (m < best_m) {
+          best_m = m;
+          best_sizes = sz;
+        }
       }
-    }
-  }
-  if (best_nb == 0 || best_m > 0.9 * m0) return SITING_OK;
-  std::vector<int> cls;
-  schedule(best_nb, best_sb, &cls);
+  if (best_sizes.empty() || best_m > 0.9 * m0) return SITING_OK;
+  std::vector<int> slot_of;
+  schedule(best_sizes, &slot_of);
+  // one class (queue) per distinct slot size, in the slots' order
+  std::vector<int> class_size;
   QueuePlan p;
-  const int nsmall = (total - best_nb * best_sb) / b->slot;
-  for (int k = 0; k < best_nb; k++) {
-    p.start.push_back(k * best_sb);
-    p.size.push_back(best_sb);
-    p.cls.push_back(0);
+  int start = 0;
+  for (int sz : best_sizes) {
+    int c = 0;
+    while (c < (int)class_size.size() && class_size[(size_t)c] != sz) c++;
+    if (c == (int)class_size.size()) class_size.push_back(sz);
+    p.start.push_back(start);
+    p.size.push_back(sz);
+    p.cls.push_back(c);
+    start += sz;
   }
-  for (int k = 0; k < nsmall; k++) {
-    p.start.push_back(best_nb * best_sb + k * b->slot);
-    p.size.push_back(b->slot);
-    p.cls.push_back(1);
-  }
+  std::vector<std::vector<int>> queues(class_size.size());
   std::vector<int> cap((size_t)njobs);
-  std::vector<int> q0, q1;
   for (int i : by_time) {
-    (cls[(size_t)i] == 0 ? q0 : q1).push_back(i);
-    cap[(size_t)i] = cls[(size_t)i] == 0 ? best_sb : b->slot;
+    int c = 0;
+    while (class_size[(size_t)c] != slot_of[(size_t)i]) c++;
+    queues[(size_t)c].push_back(i);
+    cap[(size_t)i] = slot_of[(size_t)i];
   }
-  p.off = {0, (int)q0.size(), (int)(q0.size() + q1.size())};
-  p.order = q0;
-  p.order.insert(p.order.end(), q1.begin(), q1.end());
+  p.off = {0};
+  for (const auto &q : queues) {
+    p.order.insert(p.order.end(), q.begin(), q.end());
+    p.off.push_back((int)p.order.size());
+  }
   batch_drop_graph(b);
   return batch_set_plan(b, p, cap);
 }
@@ -886,9 +902,10 @@ siting_status siting_batch_create(int device, int njobs, const siting_params *pa
   SITING_HIP(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device));
   const int cus = b->cus;
   const int nstreams = opts && opts->streams > 0 ? opts->streams : 4;
-  // 16 workgroups per slot measured best on the uneven 64-job batch
-  // (8: 640 ms, 16: 499, 32: 525, 64: 692)
-  b->slot = opts && opts->slot > 0 ? opts->slot : 16;
+  // 8 workgroups per base slot measured best on the uneven 64-job batch
+  // (step ms with re-planned slots: 4: 328, 6: 313, 8: 290, 12: 295,
+  // 16: 295, 24: 307)
+  b->slot = opts && opts->slot > 0 ? opts->slot : 8;
   if (b->slot > cus) b->slot = cus;
   b->nslots = opts && opts->nslots > 0 ? opts->nslots : cus / b->slot;
   if (b->nslots * b->slot > cus) b->nslots = cus / b->slot;
@@ -925,9 +942,10 @@ siting_status siting_batch_create(int device, int njobs, const siting_params *pa
   // nslots slots (big slots only replace base ones) and two classes
   const int max_slots = b->nslots;
   if ((e = hipMalloc(&b->args_dev, sizeof(SiteArgs) * (size_t)njobs)) != hipSuccess ||
-      (e = hipMalloc(&b->plan_dev, sizeof(int) * site_queue_plan_ints(max_slots, 2, njobs))) !=
+      (e = hipMalloc(&b->plan_dev,
+                     sizeof(int) * site_queue_plan_ints(max_slots, kMaxClasses, njobs))) !=
           hipSuccess ||
-      (e = hipMalloc(&b->ctl, 4 * site_queue_ctl_words(max_slots, 2))) != hipSuccess ||
+      (e = hipMalloc(&b->ctl, 4 * site_queue_ctl_words(max_slots, kMaxClasses))) != hipSuccess ||
       (e = hipMalloc(&b->jobt, sizeof(unsigned long long) * 2 * (size_t)njobs)) != hipSuccess)
     return fail(hip_fail(e, "hipMalloc(batch)", __FILE__, __LINE__));
   siting_status st = batch_set_plan(b, uniform_plan(b), std::vector<int>((size_t)njobs, b->slot));
