@@ -76,7 +76,7 @@ namespace {
 
 struct Layout {
   size_t elevs, elevs_t, vix, obs, sheds, area, used, cum, rowmax, order, state, recip,
-      jump, rdiv, scratch, total;
+      jump, vstab, scratch, total;
 };
 
 size_t round_up(size_t v) { return (v + 255) / 256 * 256; }
@@ -103,7 +103,7 @@ Layout layout_for(const siting_params &p, const siting_derived &d) {
   put(L.state, sizeof(SiteState));
   put(L.recip, ((size_t)p.roi + 1) * 4);
   put(L.jump, 4 * (size_t)p.ntests * 4);
-  put(L.rdiv, rdiv_entries(p.roi) * 8);
+  put(L.vstab, vs_tables(p.roi).bytes);
   put(L.scratch, site_scratch_bytes(p.nrows, p.roi, d.nwanted));
   L.total = off;
   return L;
@@ -232,18 +232,18 @@ siting_status job_create_in(int device, const siting_params *p, void *mem,
   j->site.scratch = b + L.scratch;
   j->recip = (float *)(b + L.recip);
   j->jump = (uint32_t *)(b + L.jump);
-  j->rdiv = (double *)(b + L.rdiv);
+  j->vstab = b + L.vstab;
   std::vector<float> rec((size_t)p->roi + 1);
   std::vector<uint32_t> jmp(4 * (size_t)p->ntests);
-  std::vector<double> rdv(rdiv_entries(p->roi), 0.0);
+  std::vector<uint8_t> vst(vs_tables(p->roi).bytes);
   fill_recip_table(p->roi, rec.data());
   fill_jump_table(p->ntests, jmp.data());
-  fill_rdiv_table(p->roi, rdv.data());
+  fill_vs_tables(p->roi, vst.data());
   e = hipMemcpy(j->recip, rec.data(), rec.size() * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(j->jump, jmp.data(), jmp.size() * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess)
-    e = hipMemcpy(j->rdiv, rdv.data(), rdv.size() * 8, hipMemcpyHostToDevice);
+    e = hipMemcpy(j->vstab, vst.data(), vst.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) return fail(hip_fail(e, "hipMemcpy(tables)", __FILE__, __LINE__));
   *out = j;
   return SITING_OK;
@@ -309,7 +309,7 @@ siting_status siting_job_findmax(siting_job *j) {
   JOB_STAGE(1, launch_findmax(j->g, j->vix, j->obs, j->stream));
 }
 siting_status siting_job_viewshed(siting_job *j) {
-  JOB_STAGE(2, launch_viewshed(j->g, j->elevs, j->elevs_t, j->obs, j->sheds, j->rdiv,
+  JOB_STAGE(2, launch_viewshed(j->g, j->elevs, j->elevs_t, j->obs, j->sheds, j->vstab,
                                j->stream));
 }
 siting_status siting_job_site(siting_job *j) {
@@ -527,14 +527,14 @@ siting_status siting_calc_sheds(int nrows, const uint16_t *elevs, int roi, int o
   CHECK(dalloc(det, n2 * 2));
   CHECK(dalloc(dobs, (size_t)nsheds * 4));
   CHECK(dalloc(dsh, sbytes));
-  CHECK(dalloc(drd, rdiv_entries(roi) * 8));
-  std::vector<double> rdv(rdiv_entries(roi), 0.0);
-  fill_rdiv_table(roi, rdv.data());
+  std::vector<uint8_t> vst(vs_tables(roi).bytes);
+  CHECK(dalloc(drd, vst.size()));
+  fill_vs_tables(roi, vst.data());
   SITING_HIP(hipMemcpyAsync(de.p, elevs, n2 * 2, hipMemcpyHostToDevice, s));
   SITING_HIP(hipMemcpyAsync(dobs.p, obs, (size_t)nsheds * 4, hipMemcpyHostToDevice, s));
-  SITING_HIP(hipMemcpyAsync(drd.p, rdv.data(), rdv.size() * 8, hipMemcpyHostToDevice, s));
+  SITING_HIP(hipMemcpyAsync(drd.p, vst.data(), vst.size(), hipMemcpyHostToDevice, s));
   CHECK(launch_viewshed(g, (const uint16_t *)de.p, (uint16_t *)det.p, (const int32_t *)dobs.p,
-                        (uint64_t *)dsh.p, (const double *)drd.p, s));
+                        (uint64_t *)dsh.p, (const uint8_t *)drd.p, s));
   SITING_HIP(hipMemcpyAsync(sheds, dsh.p, sbytes, hipMemcpyDeviceToHost, s));
   SITING_HIP(hipStreamSynchronize(s));
   return SITING_OK;
@@ -696,7 +696,7 @@ siting_status batch_enqueue_prep(siting_batch *b) {
     if ((st = launch_vix(j->g, j->elevs, j->vix, VixTables{j->recip, j->jump}, s)) != SITING_OK)
       return st;
     if ((st = launch_findmax(j->g, j->vix, j->obs, s)) != SITING_OK) return st;
-    if ((st = launch_viewshed(j->g, j->elevs, j->elevs_t, j->obs, j->sheds, j->rdiv, s)) !=
+    if ((st = launch_viewshed(j->g, j->elevs, j->elevs_t, j->obs, j->sheds, j->vstab, s)) !=
         SITING_OK)
       return st;
     SiteArgs a;

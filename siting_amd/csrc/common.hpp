@@ -110,16 +110,43 @@ siting_status launch_vix(const Geometry &g, const uint16_t *elevs, uint8_t *vix,
                          const VixTables &t, hipStream_t s);
 siting_status launch_findmax(const Geometry &g, uint8_t *vix, int32_t *obs,
                              hipStream_t s);
-// rdiv: 1.0/a in double for a = 0..roi (entry 0 unused), then zeros up to
-// rdiv_entries(roi): the kernel reads the table with scalar loads four steps
-// at a time, and a scalar load under a lane-divergent condition may run for
-// a step that no lane takes (at most three past roi).
-inline size_t rdiv_entries(int roi) { return (size_t)roi + 4; }
+// VIEWSHED's per-roi tables, one block of vs_tables(roi).bytes bytes.  The
+// walk takes a ray's steps in groups of four, group g holding steps
+// a = 4g+1..4g+4, g = 0..groups-1 (it reads group g+1's offsets while it
+// runs group g, so one group past the last that holds a step <= roi).
+//   off   per group, per signed minor coordinate m = -roi..roi of a ray's
+//         perimeter cell (major coordinate +-roi), four uint32, one per
+//         step: 2 (o + roi), o = the ray's minor offset at step a,
+//         sign(m) * round-half-away(fl(a * fl(|m| / roi)))
+//         (siting.cpp:323-330); 2 roi past roi
+//   step  per ray class (major axis y or x, times direction -1 or +1) and
+//         group, a VsStep4 of the group's wavefront-uniform values
+struct VsStep4 {
+  double rd[4];      // 1.0 / a (0 past roi)
+  float fa[4];       // (float)a
+  uint32_t wofs[4];  // split bitsets: byte offset of the word of major
+                     // coordinate c = roi + dir * a within a row, plus the
+                     // transposed bitset's base for x-major rays (0 past roi)
+  uint32_t wbit[4];  // its bit, 1 << (c & 31) (0 past roi)
+};
+struct VsTables {
+  size_t off, step, bytes;
+  int groups;
+};
+__host__ __device__ inline VsTables vs_tables(int roi) {
+  VsTables t;
+  t.groups = (roi - 1) / 4 + 2;
+  t.off = 0;
+  t.step = (size_t)t.groups * (2 * (size_t)roi + 1) * 16;
+  t.bytes = t.step + 4 * (size_t)t.groups * sizeof(VsStep4);
+  return t;
+}
+void fill_vs_tables(int roi, void *out);  // vs_tables(roi).bytes bytes
 // elevs_t: n * n scratch that receives the DEM transposed (the rays whose
 // major axis is y read it, so that a wavefront's reads share rows).
 siting_status launch_viewshed(const Geometry &g, const uint16_t *elevs, uint16_t *elevs_t,
                               const int32_t *obs, uint64_t *sheds,
-                              const double *rdiv, hipStream_t s);
+                              const uint8_t *tables, hipStream_t s);
 
 struct SiteBuffers {
   int32_t *area;      // [nwanted] testshedarea
@@ -204,6 +231,5 @@ siting_status site_launch_queue(const SiteArgs *jobs_dev, const int *plan_dev,
 // host tables for VIX
 void fill_recip_table(int roi, float *out);               // roi+1 entries
 void fill_jump_table(int ntests, uint32_t *out);          // 4*ntests entries
-void fill_rdiv_table(int roi, double *out);               // roi+1 entries
 
 }  // namespace siting

@@ -17,7 +17,7 @@ struct siting_job {
   siting::SiteBuffers site{};
   float *recip = nullptr;
   uint32_t *jump = nullptr;
-  double *rdiv = nullptr;
+  uint8_t *vstab = nullptr;  // VIEWSHED tables (vs_tables)
   hipEvent_t ev_start[4] = {};
   hipEvent_t ev_end[4] = {};
   bool ran[4] = {};          // stage events recorded

@@ -19,6 +19,7 @@
 // Observers are assigned XCD-major (workgroup b runs on XCD b%8, each XCD
 // takes a contiguous run of observers, which FINDMAX emitted block by block,
 // i.e. spatially adjacent), so each XCD's L2 serves one band of the DEM.
+#include <cmath>
 #include <cstdlib>
 #include <type_traits>
 
@@ -74,9 +75,8 @@ __device__ __forceinline__ int wave_min_i32(int v) {
   return v;
 }
 
-// The major axis a wavefront's rays walk, in one direction (kAxisMixed:
-// each lane's own).
-enum { kAxisY = 0, kAxisX = 1, kAxisMixed = 2 };
+// The major axis a wavefront's rays walk (all in one direction).
+enum { kAxisY = 0, kAxisX = 1 };
 
 // max(x, y) as one v_max_f32 (fmaxf adds a canonicalising max of the
 // running value; no operand here is ever NaN)
@@ -89,6 +89,9 @@ __device__ __forceinline__ float fmax_plain(float x, float y) {
 // Word 3 of a raw buffer resource on gfx9-family parts (32-bit data format,
 // no swizzle).
 constexpr int kBufferFlags = 0x00020000;
+
+// a lane's four offset-table entries of one group
+typedef uint32_t VsOff4 __attribute__((ext_vector_type(4)));
 
 
 __device__ __forceinline__ int wave_max_i32(int v) {
@@ -104,7 +107,7 @@ template <int kMode>
 __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
     const uint16_t *__restrict__ elevs, const uint16_t *__restrict__ elevs_t,
     const int32_t *__restrict__ obs,
-    uint64_t *__restrict__ sheds, const double *__restrict__ rd, int n, int roi,
+    uint64_t *__restrict__ sheds, const uint8_t *__restrict__ tabs, int n, int roi,
     int oht, int tht, int nwpr, int nsheds, int per_xcd) {
   const int nthreads = blockDim.x;
   extern __shared__ uint64_t lds_dyn[];  // the bitset(s) (kLds, kSplit)
@@ -124,6 +127,17 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
   if (kMode == kGlobal) __threadfence();
   __syncthreads();
 
+  const VsTables tb = vs_tables(roi);
+  const int cols = 2 * roi + 1;
+  // the offset table as a buffer: group g at byte 16 g cols (the scalar
+  // offset), a lane's four entries at 16 (m + roi) (the vector one)
+  const __amdgpu_buffer_rsrc_t tres = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(tabs + tb.off), (short)0, (int)(tb.step - tb.off), kBufferFlags);
+  // 2 ps in a vector register: the bit address t * 2ps + wofs is then one
+  // multiply-add with a single scalar operand
+  int ps2v;
+  asm("v_mov_b32 %0, %1" : "=v"(ps2v) : "s"(2 * ps));
+
   const int o = obs[k];
   const int ox = o / n, oy = o - (o / n) * n;
   const int oalt = (int)elevs[o] + oht;
@@ -137,28 +151,36 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
       atomicOr((unsigned long long *)&bits[roi * nwpr + (roi >> 6)], 1ull << (63 - (roi & 63)));
   }
 
-  // Rays are dealt in passes of nthreads, the y-major ones first (the
-  // interior cells of the rows dy = -roi and dy = roi), then the x-major ones
-  // (the columns dx = -roi and dx = roi, corners included), so that every
-  // wavefront but one walks a single major axis.  Lanes past the last ray
-  // repeat it, OR'ing the same bits.  A wavefront walks its rays' steps
-  // together: the step index is uniform across it and 1.0/step comes from
-  // the table through scalar loads.
-  const int ny = 4 * roi - 2;
-  for (int base = 0; base < 8 * roi; base += nthreads) {
-    const int ray = min(base + (int)threadIdx.x, 8 * roi - 1);
+  // The rays are dealt by side of the box, each side in whole wavefronts:
+  // the rows dy = -roi and dy = roi without their corners (2roi-1 y-major
+  // rays each), then the columns dx = -roi and dx = roi with them (2roi+1
+  // x-major rays each).  A wavefront's rays therefore share major axis and
+  // direction.  A lane past the end of its side repeats the ray 64 places
+  // back (walked by the wavefront before, so that the two never OR into one
+  // word in one instruction), or on a side of fewer than 64 rays one of its
+  // own; the repeated bits change nothing.  Wavefront wv of the 2(wy+wx)
+  // goes to wavefront wv mod (nthreads/64) of the workgroup.  A wavefront
+  // walks its rays' steps together, so the step index is uniform across it
+  // and the step's constants come from the tables through scalar loads.
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int my = 2 * roi - 1, mx = 2 * roi + 1;
+  const int wy = (my + kWave - 1) / kWave, wx = (mx + kWave - 1) / kWave;
+  for (int wv = wid; wv < 2 * (wy + wx); wv += nthreads / kWave) {
+    const bool along_y = wv < 2 * wy;  // |dx| < |dy| exactly for these
+    const int h = along_y ? wv : wv - 2 * wy, wside = along_y ? wy : wx;
+    const bool lo = h < wside;
+    const int m = along_y ? my : mx;
+    const int j0 = (lo ? h : h - wside) * kWave + lane;
+    const int j = j0 < m ? j0 : m >= kWave ? j0 - kWave : j0 % m;
     int dx, dy;
-    if (ray < ny) {
-      const bool lo = ray < 2 * roi - 1;
-      dx = -roi + 1 + (lo ? ray : ray - (2 * roi - 1));
+    if (along_y) {
+      dx = -roi + 1 + j;
       dy = lo ? -roi : roi;
     } else {
-      const int r = ray - ny;
-      const bool lo = r < 2 * roi + 1;
       dx = lo ? -roi : roi;
-      dy = -roi + (lo ? r : r - (2 * roi + 1));
+      dy = -roi + j;
     }
-    const bool along_y = ray < ny;  // |dx| < |dy| exactly for these
     const int major = along_y ? dy : dx;
     const int minor = along_y ? dx : dy;
     const int len = major < 0 ? -major : major;
@@ -197,17 +219,17 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
     float horizon = -99999.0f;  // siting.cpp:325
     const int wmin = __builtin_amdgcn_readfirstlane(wave_min_i32(last));
     const int wlast = __builtin_amdgcn_readfirstlane(wave_max_i32(last));
+    // the lane's entries in the offset table (bytes)
+    const int tcol = 16 * (minor + roi);
 
-    // kCls: kAxisY or kAxisX when every lane of the wavefront walks that
-    // major axis in one direction (the step's major coordinate, the DEM row
-    // it reads and its bit position are then wavefront-uniform), else
-    // kAxisMixed (each lane selects).
+    // kCls: the major axis every lane of the wavefront walks, in one
+    // direction, so the step's major coordinate, the DEM row it reads and
+    // its bit position are wavefront-uniform.
     auto walk = [&](auto cls_c) {
-      constexpr int kCls = decltype(cls_c)::value;
-      constexpr bool kUni = kCls != kAxisMixed;
-      const bool ay = kCls == kAxisY ? true : kCls == kAxisX ? false : along_y;
-      const int sdir = kUni ? __builtin_amdgcn_readfirstlane(dir) : dir;
-      const int sps4 = sgn * ps * 4;
+      constexpr bool ay = decltype(cls_c)::value == kAxisY;
+      const int sdir = __builtin_amdgcn_readfirstlane(dir);
+      const VsStep4 *rec =
+          (const VsStep4 *)(tabs + tb.step) + (size_t)((ay ? 0 : 2) + (sdir > 0)) * tb.groups;
       // siting.cpp:337-344 for one step.  The point slope is siting.cpp:340,
       // (float)(pel - oalt) / (float)a: with |pel - oalt| < 2^21 and
       // a < 2^14 the quotient is never closer than 2^-39 (relative) to a
@@ -216,90 +238,100 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
       // correctly rounded fp32 quotient, bit for bit.  (The running maximum
       // compares as the reference's "if (horizon < s) horizon = s": the
       // slopes are never NaN and the sign of a zero horizon changes
-      // nothing downstream.)
-      auto step = [&](bool take, int a, float fa, double rda, int pel, int offa) {
+      // nothing downstream.)  t: the offset table's entry 2 (off + roi).
+      auto step = [&](bool take, int a, float fa, double rda, uint32_t wofs, uint32_t wbit,
+                      int pel, int t) {
         const float sl = (float)((double)(pel - oalt) * rda);
         horizon = take ? fmax_plain(horizon, sl) : horizon;
         const float rise = __fmul_rn(horizon, fa);
         const float halt = __fadd_rn(foalt, rise);  // siting.cpp:342
-        if (take && (float)(pel + tht) >= halt) {
+        const bool vis = take && (float)(pel + tht) >= halt;
+        if (kMode == kSplit && vis) {
+          // y-major: row off + roi, column c; x-major, transposed: column
+          // off + roi, row c; the row's byte offset (off + roi) 4 ps is t 2ps
+          atomicOr((uint32_t *)((uint8_t *)b32 + wofs + __umul24(t, ps2v)), wbit);
+        } else if (kMode != kSplit && vis) {
           // (24-bit products: rows, columns and strides are below 2^15)
-          // the major coordinate in the box (wavefront-uniform in kUni)
-          const int c = kUni ? __builtin_amdgcn_readfirstlane(roi + sdir * a) : roi + sdir * a;
-          if (kMode == kSplit && kUni) {
-            // y-major: row off + roi, column c; x-major, transposed: column
-            // off + roi, row c
-            const int wb = 4 * ((ay ? roi * ps : nr * ps + roi * ps) + (c >> 5));
-            atomicOr((uint32_t *)((uint8_t *)b32 + wb + __mul24(offa, sps4)), 1u << (c & 31));
-          } else {
-            const int off = __mul24(sgn, offa);
-            const int row = ay ? off + roi : c, col = ay ? c : off + roi;
-            if (kMode == kSplit) {
-              const int w = ay ? (int)__umul24(row, ps) + (col >> 5)
-                               : nr * ps + (int)__umul24(col, ps) + (row >> 5);
-              atomicOr(&b32[w], 1u << ((ay ? col : row) & 31));
-            } else {
-              atomicOr((unsigned long long *)&bits[(int)__umul24(row, nwpr) + (col >> 6)],
-                       1ull << (63 - (col & 63)));
-            }
-          }
+          const int c = __builtin_amdgcn_readfirstlane(roi + sdir * a);
+          const int row = ay ? t >> 1 : c, col = ay ? c : t >> 1;
+          atomicOr((unsigned long long *)&bits[(int)__umul24(row, nwpr) + (col >> 6)],
+                   1ull << (63 - (col & 63)));
         }
       };
       // Four steps at a time, their DEM loads issued before any is used.
-      // The four reciprocals are read first (scalar loads share their wait
+      // Their minor offsets come from the offset table (one 16-byte load
+      // per group, no arithmetic); the next group's are loaded while this
+      // group's steps run.  The group's uniform values are read with scalar
+      // loads after the DEM loads are issued (scalar loads share their wait
       // counter with the LDS atomics, so a read just before its use would
-      // also wait for every atomic in flight; the table has three spare
-      // entries past roi for the last group).  The y-major rays read the
+      // also wait for every atomic in flight).  The y-major rays read the
       // transposed DEM, so that a wavefront's reads share a row either way
-      // (in a uniform wavefront, the one row).  DEM offsets are 32-bit with
-      // 24-bit products (n < 2^15).
+      // (the one row).
       const uint16_t *dem = ay ? elevs_t : elevs;
       const int omaj = ay ? oy : ox, omin = ay ? ox : oy;
-      const int sgn2 = 2 * sgn;
-      // the DEM as a buffer of 2 n^2 bytes (raw, no stride)
-      const __amdgpu_buffer_rsrc_t dres =
-          __builtin_amdgcn_make_buffer_rsrc((void *)dem, (short)0, 2 * n * n, kBufferFlags);
-      auto group = [&](int a, auto checked_c) {
+      // The DEM as a raw buffer whose byte 0 is column omin - roi of row 0:
+      // a cell's row is the scalar offset and the table entry 2 (off + roi)
+      // the vector one.  Every access is a cell of the terrain (the walk
+      // stops at its edge), so the base may lie before the DEM's first cell
+      // without anything being read there.
+      const __amdgpu_buffer_rsrc_t dres = __builtin_amdgcn_make_buffer_rsrc(
+          (void *)(dem + (omin - roi)), (short)0, 2 * n * n + 4 * roi, kBufferFlags);
+      auto tload = [&](int g) {
+        return __builtin_amdgcn_raw_buffer_load_b128(tres, tcol, g * (16 * cols), 0);
+      };
+      // byte offset of the DEM row of step 1, and from one step to the next
+      const int sd = 2 * n * sdir;
+      int srow = 2 * n * omaj + sd;
+      // group g: steps 4g+1..4g+4; tc its table entries, tn receives the
+      // next group's
+      auto group = [&](int g, auto checked_c, const VsOff4 &tc, VsOff4 &tn) {
         constexpr bool kChecked = decltype(checked_c)::value;
-        double rda[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) rda[u] = rd[a + u];
-        int offa[4], pel[4];
+        const int a = 4 * g + 1;
+        int pel[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-          offa[u] = offset_abs((float)(a + u));
           int v = 0;
-          if (!kChecked || a + u <= last) {
-            if (kUni) {
-              // a buffer load: the row's byte offset as the scalar offset,
-              // the lane's cell in the row as the vector one (both >= 0)
-              const int soff = 2 * (int)__umul24((uint32_t)(omaj + sdir * (a + u)), (uint32_t)n);
-              v = __builtin_amdgcn_raw_buffer_load_b16(dres, __mul24(offa[u], sgn2) + 2 * omin, soff, 0);
-            } else {
-              const uint32_t prow = (uint32_t)(omaj + sdir * (a + u));
-              const uint32_t pcol = (uint32_t)(omin + __mul24(sgn, offa[u]));
-              v = dem[__umul24(prow, (uint32_t)n) + pcol];
-            }
-          }
+          if (!kChecked || a + u <= last)
+            v = __builtin_amdgcn_raw_buffer_load_b16(dres, tc[u], srow + u * sd, 0);
           pel[u] = v;
         }
-        // (no instruction: makes the four reciprocals ready here, after the
-        // DEM loads are issued and before the steps' conditional blocks, so
-        // that none of those waits again)
-        asm volatile("" ::"s"(rda[0]), "s"(rda[1]), "s"(rda[2]), "s"(rda[3]));
+        // (unconditional, so that the waits for this group's DEM loads let
+        // it stay in flight)
+        tn = tload(g + 1);
+        const VsStep4 r = rec[g];
+        // (no instruction: makes the scalar values ready here, before the
+        // steps' conditional blocks, so that none of those waits again)
+        asm volatile("" ::"s"(r.rd[0]), "s"(r.rd[1]), "s"(r.rd[2]), "s"(r.rd[3]),
+                     "s"(r.fa[0]), "s"(r.fa[1]), "s"(r.fa[2]), "s"(r.fa[3]), "s"(r.wofs[0]),
+                     "s"(r.wofs[1]), "s"(r.wofs[2]), "s"(r.wofs[3]), "s"(r.wbit[0]),
+                     "s"(r.wbit[1]), "s"(r.wbit[2]), "s"(r.wbit[3]));
 #pragma unroll
         for (int u = 0; u < 4; u++)
-          step(!kChecked || a + u <= last, a + u, (float)(a + u), rda[u], pel[u], offa[u]);
+          step(!kChecked || a + u <= last, a + u, r.fa[u], r.rd[u], r.wofs[u], r.wbit[u],
+               pel[u], (int)tc[u]);
+        srow += 4 * sd;
       };
-      int a = 1;
-      // steps every lane takes, then the rest, each lane up to its own last
-      for (; a + 3 <= wmin; a += 4) group(a, std::integral_constant<bool, false>{});
-      for (; a <= wlast; a += 4) group(a, std::integral_constant<bool, true>{});
+      // steps every lane takes, then the rest, each lane up to its own last;
+      // the unchecked groups in pairs that swap the two entry buffers, the
+      // others with the next entries loaded into ta and this group's copied
+      const std::integral_constant<bool, false> unchecked;
+      const std::integral_constant<bool, true> checked;
+      VsOff4 ta = tload(0), tb4;
+      int g = 0;
+      for (; 4 * g + 8 <= wmin; g += 2) {
+        group(g, unchecked, ta, tb4);
+        group(g + 1, unchecked, tb4, ta);
+      }
+      for (; 4 * g + 4 <= wmin; g++) {
+        const VsOff4 tc = ta;
+        group(g, unchecked, tc, ta);
+      }
+      for (; 4 * g + 1 <= wlast; g++) {
+        const VsOff4 tc = ta;
+        group(g, checked, tc, ta);
+      }
     };
-    const int cls = (along_y ? 0 : 2) + (dir > 0 ? 1 : 0);
-    if (__ballot(cls == __builtin_amdgcn_readfirstlane(cls)) != ~0ull)
-      walk(std::integral_constant<int, kAxisMixed>{});
-    else if (along_y)
+    if (along_y)
       walk(std::integral_constant<int, kAxisY>{});
     else
       walk(std::integral_constant<int, kAxisX>{});
@@ -351,14 +383,48 @@ __global__ __launch_bounds__(kTile * kTileRows) void dem_transpose(
 
 }  // namespace
 
-void fill_rdiv_table(int roi, double *out) {
-  out[0] = 0.0;
-  for (int a = 1; a <= roi; a++) out[a] = 1.0 / (double)a;
+void fill_vs_tables(int roi, void *out) {
+  const VsTables t = vs_tables(roi);
+  uint8_t *b = (uint8_t *)out;
+  uint32_t *off = (uint32_t *)(b + t.off);
+  VsStep4 *rec = (VsStep4 *)(b + t.step);
+  const int cols = 2 * roi + 1, nr = 2 * roi + 1;
+  const int ps = split_stride((nr + 63) / 64);
+  for (int g = 0; g < t.groups; g++) {
+    for (int u = 0; u < 4; u++) {
+      const int a = 4 * g + 1 + u;
+      for (int m = -roi; m <= roi; m++) {
+        int o = 0;
+        if (a <= roi) {
+          // the kernel's ray slope is fl(minor / major), |major| = roi; the
+          // offset's magnitude is fl(a |slope|) rounded half away from zero,
+          // its sign that of the minor coordinate (on every side of the box)
+          const float aslope = std::fabs((float)m / (float)roi);
+          const volatile float prod = (float)a * aslope;  // one rounding each
+          const volatile float biased = prod + kHalfDown;
+          o = (int)biased;
+          if (m < 0) o = -o;
+        }
+        off[((size_t)g * cols + (m + roi)) * 4 + u] = (uint32_t)(2 * (o + roi));
+      }
+      for (int cls = 0; cls < 4; cls++) {
+        const bool ay = cls < 2;
+        const int dir = (cls & 1) ? 1 : -1;
+        VsStep4 &r = rec[(size_t)cls * t.groups + g];
+        const bool in = a <= roi;
+        const int c = roi + dir * a;
+        r.rd[u] = in ? 1.0 / (double)a : 0.0;
+        r.fa[u] = (float)a;
+        r.wofs[u] = in ? 4u * (uint32_t)((ay ? 0 : nr * ps) + (c >> 5)) : 0u;
+        r.wbit[u] = in ? 1u << (c & 31) : 0u;
+      }
+    }
+  }
 }
 
 siting_status launch_viewshed(const Geometry &g, const uint16_t *elevs, uint16_t *elevs_t,
                               const int32_t *obs, uint64_t *sheds,
-                              const double *rdiv, hipStream_t s) {
+                              const uint8_t *tables, hipStream_t s) {
   const int nsheds = g.d.nwanted;
   if (nsheds <= 0) return SITING_OK;
   const int tiles = (g.p.nrows + kTile - 1) / kTile;
@@ -379,7 +445,7 @@ siting_status launch_viewshed(const Geometry &g, const uint16_t *elevs, uint16_t
                      : mode == kLds ? sizeof(uint64_t) * words : 0;
 #define SITING_VS(A)                                                               \
   hipLaunchKernelGGL((viewshed_kernel<A>), dim3(8 * per_xcd), dim3(threads), lds, s, \
-                     elevs, elevs_t, obs, sheds, rdiv, g.p.nrows, g.p.roi, g.p.oht,    \
+                     elevs, elevs_t, obs, sheds, tables, g.p.nrows, g.p.roi, g.p.oht,  \
                      g.p.tht, g.d.nwpr, nsheds, per_xcd)
   if (mode == kSplit) SITING_VS(kSplit);
   else if (mode == kLds) SITING_VS(kLds);

@@ -51,28 +51,41 @@ def test_float_index_split_exact():
         assert np.array_equal(q, it // nk), nk
 
 
-def _ray_cells(roi, ray):
-    """Perimeter target of ray `ray` and the ray's (major, minor, along_y, len,
-    dir, float32 slope), as VIEWSHED enumerates them (siting.cpp:292-323).
-    The kernel deals the y-major rays first (the interior cells of the rows
-    dy = -roi, dy = roi), then the x-major ones (the columns dx = -roi,
-    dx = roi, corners included): the same 8*roi perimeter cells as the
-    reference's four sides, each once."""
-    ny = 4 * roi - 2
-    if ray < ny:
-        lo = ray < 2 * roi - 1
-        dx = -roi + 1 + (ray if lo else ray - (2 * roi - 1))
-        dy = -roi if lo else roi
-    else:
-        r = ray - ny
-        lo = r < 2 * roi + 1
-        dx = -roi if lo else roi
-        dy = -roi + (r if lo else r - (2 * roi + 1))
+def _wave_rays(roi):
+    """The rays of VIEWSHED's wavefronts, as (g, lane, dx, dy, valid): each
+    side of the box is dealt in whole wavefronts of 64 lanes, the rows
+    dy = -roi and dy = roi without corners (y-major), then the columns
+    dx = -roi and dx = roi with them (x-major); lanes past the end of a side
+    repeat its last ray with valid False."""
+    my, mx = 2 * roi - 1, 2 * roi + 1
+    wy, wx = -(-my // 64), -(-mx // 64)
+    for g in range(2 * (wy + wx)):
+        along_y = g < 2 * wy
+        h = g if along_y else g - 2 * wy
+        wside = wy if along_y else wx
+        lo = h < wside
+        m = my if along_y else mx
+        for lane in range(64):
+            j0 = (h if lo else h - wside) * 64 + lane
+            j = min(j0, m - 1)
+            if along_y:
+                dx, dy = -roi + 1 + j, (-roi if lo else roi)
+            else:
+                dx, dy = (-roi if lo else roi), -roi + j
+            yield g, lane, dx, dy, j0 < m
+
+
+def _ray(roi, dx, dy):
+    """A ray's (along_y, len, dir, float32 slope) (siting.cpp:292-323)."""
     along_y = abs(dx) < abs(dy)
-    assert along_y == (ray < ny)
     major, minor = (dy, dx) if along_y else (dx, dy)
     slope = np.float32(minor) / np.float32(major)
     return along_y, abs(major), (1 if major > 0 else -1), slope
+
+
+def _ray_cells(roi):
+    """The valid rays of the kernel's wavefronts, in order."""
+    return [_ray(roi, dx, dy) for _, _, dx, dy, ok in _wave_rays(roi) if ok]
 
 
 def _offsets(dirn, slope, a):
@@ -90,13 +103,12 @@ def test_viewshed_ray_prefix_by_bisection():
     checked against the step-by-step walk for every ray of several roi, with
     observers at corners, edges and inside small terrains."""
     for roi in (1, 2, 3, 5, 7, 13, 31, 64, 100, 150):
-        rays = range(8 * roi)
+        rays = _ray_cells(roi)
         for n in (1, 2, 7, 40, 301):
             obs = {(0, 0), (0, n - 1), (n - 1, 0), (n - 1, n - 1), (n // 2, n // 2),
                    (0, n // 2), (n // 3, n - 1), (min(n - 1, roi), min(n - 1, 2))}
             for ox, oy in obs:
-                for ray in rays:
-                    along_y, length, dirn, slope = _ray_cells(roi, ray)
+                for ray, (along_y, length, dirn, slope) in enumerate(rays):
                     a = np.arange(1, length + 1, dtype=np.int64)
                     off = _offsets(dirn, slope, a)
                     i = dirn * a
@@ -117,12 +129,21 @@ def test_viewshed_ray_prefix_by_bisection():
 
 
 def test_ray_set_is_the_box_perimeter():
-    """The kernel's ray order covers every perimeter cell of the (2roi+1)^2
-    box exactly once, as the reference's four sides do (siting.cpp:292-304)."""
-    for roi in (1, 2, 3, 25, 100, 150):
+    """The kernel's valid lanes cover every perimeter cell of the (2roi+1)^2
+    box exactly once, as the reference's four sides do (siting.cpp:292-304);
+    every wavefront walks one major axis in one direction; a padding lane
+    repeats a valid ray of its own wavefront."""
+    for roi in (1, 2, 3, 25, 31, 32, 33, 50, 100, 150, 333):
+        waves = {}
+        for g, lane, dx, dy, ok in _wave_rays(roi):
+            along_y, length, dirn, _ = _ray(roi, dx, dy)
+            waves.setdefault(g, []).append(((along_y, dirn), (dx, dy), ok))
+        for lanes in waves.values():
+            assert len(lanes) == 64 and len({c for c, _, _ in lanes}) == 1
+            assert lanes[0][2]
+            assert {r for _, r, ok in lanes if not ok} <= {r for _, r, ok in lanes if ok}
         cells = []
-        for ray in range(8 * roi):
-            along_y, length, dirn, slope = _ray_cells(roi, ray)
+        for along_y, length, dirn, slope in _ray_cells(roi):
             end = _offsets(dirn, slope, np.array([length]))[0]
             cells.append((int(end), dirn * length) if along_y else (dirn * length, int(end)))
         want = {(x, y) for x in range(-roi, roi + 1) for y in range(-roi, roi + 1)
@@ -150,8 +171,7 @@ def test_half_away_rounding_by_biased_truncation():
         bad = np.flatnonzero(got != want)
         assert bad.size == 0, float(x[bad[0]])
     for roi in (1, 2, 7, 25, 100, 150, 333):
-        for ray in range(8 * roi):
-            along_y, length, dirn, slope = _ray_cells(roi, ray)
+        for ray, (along_y, length, dirn, slope) in enumerate(_ray_cells(roi)):
             a = np.arange(1, length + 1, dtype=np.int64)
             ref = _offsets(dirn, slope, a)
             sgn = 1 if (dirn > 0) == (slope >= 0) else -1
