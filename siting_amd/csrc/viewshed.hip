@@ -92,6 +92,15 @@ constexpr int kBufferFlags = 0x00020000;
 
 // a lane's four offset-table entries of one group
 typedef uint32_t VsOff4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) uint32_t LdsU32;
+// the step records, read-only for the kernel's life: the constant address
+// space, so that they are read with scalar loads (the host pass, which only
+// parses the kernel, gets a plain pointer)
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const __attribute__((address_space(4))) VsStep4 ConstStep4;
+#else
+typedef const VsStep4 ConstStep4;
+#endif
 
 
 __device__ __forceinline__ int wave_max_i32(int v) {
@@ -137,6 +146,9 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
   // multiply-add with a single scalar operand
   int ps2v;
   asm("v_mov_b32 %0, %1" : "=v"(ps2v) : "s"(2 * ps));
+  // the split bitsets as a byte pointer in the LDS address space
+  __attribute__((address_space(3))) uint8_t *lds8 =
+      (__attribute__((address_space(3))) uint8_t *)b32;
 
   const int o = obs[k];
   const int ox = o / n, oy = o - (o / n) * n;
@@ -228,8 +240,8 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
     auto walk = [&](auto cls_c) {
       constexpr bool ay = decltype(cls_c)::value == kAxisY;
       const int sdir = __builtin_amdgcn_readfirstlane(dir);
-      const VsStep4 *rec =
-          (const VsStep4 *)(tabs + tb.step) + (size_t)((ay ? 0 : 2) + (sdir > 0)) * tb.groups;
+      ConstStep4 *rec =
+          (ConstStep4 *)(tabs + tb.step) + (size_t)((ay ? 0 : 2) + (sdir > 0)) * tb.groups;
       // siting.cpp:337-344 for one step.  The point slope is siting.cpp:340,
       // (float)(pel - oalt) / (float)a: with |pel - oalt| < 2^21 and
       // a < 2^14 the quotient is never closer than 2^-39 (relative) to a
@@ -249,7 +261,8 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
         if (kMode == kSplit && vis) {
           // y-major: row off + roi, column c; x-major, transposed: column
           // off + roi, row c; the row's byte offset (off + roi) 4 ps is t 2ps
-          atomicOr((uint32_t *)((uint8_t *)b32 + wofs + __umul24(t, ps2v)), wbit);
+          __hip_atomic_fetch_or((LdsU32 *)(lds8 + wofs + __umul24(t, ps2v)), wbit,
+                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         } else if (kMode != kSplit && vis) {
           // (24-bit products: rows, columns and strides are below 2^15)
           const int c = __builtin_amdgcn_readfirstlane(roi + sdir * a);
@@ -284,7 +297,7 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
       int srow = 2 * n * omaj + sd;
       // group g: steps 4g+1..4g+4; tc its table entries, tn receives the
       // next group's
-      auto group = [&](int g, auto checked_c, const VsOff4 &tc, VsOff4 &tn) {
+      auto group = [&](int g, auto checked_c, const VsOff4 &tc, VsOff4 &tn, ConstStep4 *rg) {
         constexpr bool kChecked = decltype(checked_c)::value;
         const int a = 4 * g + 1;
         int pel[4];
@@ -298,7 +311,7 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
         // (unconditional, so that the waits for this group's DEM loads let
         // it stay in flight)
         tn = tload(g + 1);
-        const VsStep4 r = rec[g];
+        const VsStep4 r = *rg;
         // (no instruction: makes the scalar values ready here, before the
         // steps' conditional blocks, so that none of those waits again)
         asm volatile("" ::"s"(r.rd[0]), "s"(r.rd[1]), "s"(r.rd[2]), "s"(r.rd[3]),
@@ -316,19 +329,24 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
       // others with the next entries loaded into ta and this group's copied
       const std::integral_constant<bool, false> unchecked;
       const std::integral_constant<bool, true> checked;
+      // (rg: group g's record; kept opaque to the loop optimiser so that
+      // the records are read at small positive offsets from it)
       VsOff4 ta = tload(0), tb4;
+      ConstStep4 *rg = rec;
       int g = 0;
       for (; 4 * g + 8 <= wmin; g += 2) {
-        group(g, unchecked, ta, tb4);
-        group(g + 1, unchecked, tb4, ta);
+        group(g, unchecked, ta, tb4, rg);
+        group(g + 1, unchecked, tb4, ta, rg + 1);
+        rg += 2;
+        asm volatile("" : "+s"(rg));
       }
-      for (; 4 * g + 4 <= wmin; g++) {
+      for (; 4 * g + 4 <= wmin; g++, rg++) {
         const VsOff4 tc = ta;
-        group(g, unchecked, tc, ta);
+        group(g, unchecked, tc, ta, rg);
       }
-      for (; 4 * g + 1 <= wlast; g++) {
+      for (; 4 * g + 1 <= wlast; g++, rg++) {
         const VsOff4 tc = ta;
-        group(g, checked, tc, ta);
+        group(g, checked, tc, ta, rg);
       }
     };
     if (along_y)
