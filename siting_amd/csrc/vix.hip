@@ -33,6 +33,27 @@ constexpr int win_edge() { return kTile + 2 * kHalo; }
 // The reciprocal table (roi + 1 floats) is copied to LDS up to this roi.
 constexpr int kRecipLdsMax = 2048;
 
+// x / 3 truncated toward zero (siting.cpp:164-165) as trunc(fl(x * fl(1/3)))
+// with fl(1/3) = 0x3eaaaaab, slightly above 1/3: exact for every |x| <= 2^16
+// (checked exhaustively; here |x| <= 2 roi + 1 < 2^15).
+__device__ __forceinline__ int div3(int x) {
+  return (int)__fmul_rn((float)x, 0.33333334f);
+}
+
+// Word 3 of a raw buffer resource on gfx9-family parts (32-bit data format,
+// no swizzle).
+constexpr int kBufferFlags = 0x00020000;
+
+typedef const __attribute__((address_space(3))) uint16_t LdsU16;
+
+// a * b + c with 24-bit signed a and b, as one v_mad_i32_i24 (the compiler
+// otherwise fuses the add into a 64-bit multiply-add)
+__device__ __forceinline__ int mad24(int a, int b, int c) {
+  int r;
+  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
 // siting.cpp:157,160, evaluated in float in the reference's order:
 // ((float)(2roi) + 0.99999f) * (float)r / 2^31, truncated.
 __device__ __forceinline__ int target_offset(float span, uint32_t r) {
@@ -74,6 +95,14 @@ __global__ __launch_bounds__(kThreads) void vix_kernel(
   }
   __syncthreads();
 
+  // the window's LDS byte address, folded into each pair's base so that a
+  // sample's address is a shift-add and a multiply-add
+  const int wlds = (int)(uint32_t)(uintptr_t)(const LdsU16 *)win;
+  constexpr int kHaloLog = kHalo == 8 ? 3 : kHalo == 16 ? 4 : 5;
+  static_assert(kHalo == 1 << kHaloLog, "the halo is a power of two");
+  // the DEM as a raw buffer of 2 n^2 bytes: byte offsets below 2^31
+  const __amdgpu_buffer_rsrc_t dres =
+      __builtin_amdgcn_make_buffer_rsrc((void *)elevs, (short)0, 2 * n * n, kBufferFlags);
   const float span = (float)(2 * roi) + 0.99999f;
   const int rr = roi * roi;
   const int total = kThreads * ntests;
@@ -88,18 +117,19 @@ __global__ __launch_bounds__(kThreads) void vix_kernel(
     const int py = ty0 + (lp & 15);
     bool vis = false;
     if (px < n && py < n) {
-      const uint32_t p = (uint32_t)(px * n + py);
+      // (24-bit products throughout: coordinates, n and distances are below
+      // 2^15, so every product below fits 32 bits)
+      const uint32_t p = __umul24((uint32_t)px, (uint32_t)n) + (uint32_t)py;
       const uint32_t r1 = (jl[4 * k] * p + jl[4 * k + 1]) & 0x7fffffffu;
       const uint32_t r2 = (jl[4 * k + 2] * p + jl[4 * k + 3]) & 0x7fffffffu;
-      int tx = target_offset(span, r1) + (px - roi);
-      int ty = target_offset(span, r2) + (py - roi);
-      int ddx = tx - px, ddy = ty - py;
-      if (ddx * ddx + ddy * ddy > rr) {  // siting.cpp:162-166
-        tx = px + ddx / 3;
-        ty = py + ddy / 3;
+      int dx = target_offset(span, r1) - roi;
+      int dy = target_offset(span, r2) - roi;
+      if (__mul24(dx, dx) + __mul24(dy, dy) > rr) {  // siting.cpp:162-166
+        dx = div3(dx);
+        dy = div3(dy);
       }
-      if (tx >= 0 && tx < n && ty >= 0 && ty < n) {
-        const int dx = tx - px, dy = ty - py;
+      const int tx = px + dx, ty = py + dy;
+      if ((uint32_t)tx < (uint32_t)n && (uint32_t)ty < (uint32_t)n) {
         const int adx = dx < 0 ? -dx : dx, ady = dy < 0 ? -dy : dy;
         if (adx <= 1 && ady <= 1) {
           vis = true;
@@ -112,34 +142,40 @@ __global__ __launch_bounds__(kThreads) void vix_kernel(
           if (major < 0) rec = -rec;  // (float)(1.0/-d) == -(float)(1.0/d)
           const float slope = (float)minor * rec;
           const int dir = major > 0 ? 1 : -1;
-          const int oz = (int)win[(px - wx0) * kWin + (py - wy0)] + oht;
-          const int tz = (int)elevs[tx * n + ty] + tht;
+          const int wcell = __mul24(px - wx0, kWin) + (py - wy0);
+          const int oz = (int)win[wcell] + oht;
+          const int tz = (int)elevs[__umul24((uint32_t)tx, (uint32_t)n) + (uint32_t)ty] + tht;
           const float zs = (float)(tz - oz) * rec;
           const float foz = (float)oz;
           // The reference samples id = dir*1, dir*2, dir*4, ... while |id| <
           // len and stops at the first sample above the sight line, so the
-          // target is visible iff no sample is above it.  With |id| a power
-          // of two, slope*id and id*zs are exact scalings of slope and zs:
-          // both double from one sample to the next, exactly.  The minor
-          // offset rounds slope*id half away from zero, i.e. sign *
+          // target is visible iff no sample is above it.  With |id| = 2^j,
+          // slope*id and id*zs are exact scalings of slope and zs: both
+          // double from one sample to the next, exactly.  The minor offset
+          // rounds slope*id half away from zero, i.e. sign *
           // trunc(fl(|slope*id| + (0.5 - 2^-25))).  Samples up to kHalo
-          // steps away are read from the LDS window.
+          // steps away are read from the LDS window, the rest from the DEM.
           const bool neg = (slope < 0.0f) != (dir < 0);  // sign of slope*id
           float xs = fabsf(slope);                        // |slope*id| at |id| = 1
           float ts = dir > 0 ? zs : -zs;                  // id*zs at |id| = 1
-          // cell of sample (id, off) in the window and in the DEM: base +
-          // |id| * (major step) + off * (minor step) (24-bit products: the
-          // window edge, |id|, |off| and n are all below 2^15)
-          const int wmaj = alx ? dir * kWin : dir, wmin = alx ? 1 : kWin;
-          const int gmaj = alx ? dir * n : dir, gmin = alx ? 1 : n;
-          const int wbase = (px - wx0) * kWin + (py - wy0);
-          const int gbase = px * n + py;
+          // sample j's cell, in bytes: base + (major step << j) + |off| *
+          // (signed minor step), in the window and in the DEM
+          const int w2 = 2 * kWin, g2 = 2 * n;
+          const int wmaj = alx ? (dir > 0 ? w2 : -w2) : (dir > 0 ? 2 : -2);
+          const int wmin = alx ? (neg ? -2 : 2) : (neg ? -w2 : w2);
+          const int gmaj = alx ? (dir > 0 ? g2 : -g2) : (dir > 0 ? 2 : -2);
+          const int gmin = alx ? (neg ? -2 : 2) : (neg ? -g2 : g2);
+          const int wbase = 2 * wcell + wlds;
+          const int gbase = 2 * (int)p;
+          // samples: |id| = 2^j < len, j < nsm; the first nlds in the window
+          const int nsm = 32 - __clz(len - 1);
+          const int nlds = nsm < kHaloLog + 1 ? nsm : kHaloLog + 1;
           vis = true;
-          int sv = 1;
-          for (; sv < len && sv <= kHalo; sv <<= 1) {
+          int j = 0;
+          for (; j < nlds; j++) {
             const int oa = (int)__fadd_rn(xs, kHalfDown);
-            const int off = neg ? -oa : oa;
-            const float z = (float)win[wbase + __mul24(sv, wmaj) + __mul24(off, wmin)];
+            const int cell = mad24(oa, wmin, (int)((uint32_t)wmaj << j) + wbase);
+            const float z = (float)*(const LdsU16 *)(uintptr_t)(uint32_t)cell;
             if (z > __fadd_rn(foz, ts)) {
               vis = false;
               break;
@@ -148,11 +184,11 @@ __global__ __launch_bounds__(kThreads) void vix_kernel(
             ts = __fadd_rn(ts, ts);
           }
           if (vis) {
-            for (; sv < len; sv <<= 1) {
+            for (; j < nsm; j++) {
               const int oa = (int)__fadd_rn(xs, kHalfDown);
-              const int off = neg ? -oa : oa;
-              const uint32_t cell = (uint32_t)(gbase + __mul24(sv, gmaj) + __mul24(off, gmin));
-              const float z = (float)elevs[cell];
+              const uint32_t cell =
+                  (uint32_t)mad24(oa, gmin, (int)((uint32_t)gmaj << j) + gbase);
+              const float z = (float)__builtin_amdgcn_raw_buffer_load_b16(dres, cell, 0, 0);
               if (z > __fadd_rn(foz, ts)) {
                 vis = false;
                 break;

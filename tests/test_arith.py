@@ -178,3 +178,17 @@ def test_half_away_rounding_by_biased_truncation():
             mag = (a.astype(np.float32) * np.float32(abs(slope))).astype(np.float32)
             mine = sgn * (mag + h).astype(np.float32).astype(np.int64)
             assert np.array_equal(mine, ref), (roi, ray)
+
+
+def test_div3_by_float_reciprocal():
+    """VIX pulls a far target a third of the way in with C's truncating
+    x / 3 (siting.cpp:164-165), computed as trunc(fl(x * fl(1/3))) with
+    fl(1/3) = 0x3eaaaaab (just above 1/3).  Checked for every |x| <= 2^16;
+    the kernel's x is a target offset, |x| <= 2 roi + 1 < 2^15."""
+    third = np.float32(0.33333334)
+    assert third.view(np.uint32) == 0x3EAAAAAB
+    assert np.float64(third) > 1.0 / 3.0
+    x = np.arange(-(1 << 16), (1 << 16) + 1, dtype=np.int64)
+    got = np.trunc(x.astype(np.float32) * third).astype(np.int64)
+    want = np.sign(x) * (np.abs(x) // 3)
+    assert np.array_equal(got, want)
