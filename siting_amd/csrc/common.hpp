@@ -119,8 +119,10 @@ siting_status launch_findmax(const Geometry &g, uint8_t *vix, int32_t *obs,
 //         step: 2 (o + roi), o = the ray's minor offset at step a,
 //         sign(m) * round-half-away(fl(a * fl(|m| / roi)))
 //         (siting.cpp:323-330); 2 roi past roi
-//   step  per ray class (major axis y or x, times direction -1 or +1) and
-//         group, a VsStep4 of the group's wavefront-uniform values
+//   step  per ray class (major axis y or x, times direction -1 or +1),
+//         chunk of 64 rays of a side (in the split-bitset mode; one chunk
+//         otherwise) and group, a VsStep4 of the group's wavefront-uniform
+//         values
 struct VsStep4 {
   double rd[4];      // 1.0 / a (0 past roi)
   float fa[4];       // (float)a
@@ -128,17 +130,27 @@ struct VsStep4 {
                      // coordinate c = roi + dir * a within a row, plus the
                      // transposed bitset's base for x-major rays (0 past roi)
   uint32_t wbit[4];  // its bit, 1 << (c & 31) (0 past roi)
+  uint64_t same[4];  // split bitsets: bit l set when lane l's cell at step a
+                     // is lane l-1's (0 past roi)
 };
+// The split-bitset mode: two LDS bitsets of 32-bit words, 2 nwpr + 1 dwords
+// a row, when both fit in 48 KiB.
+__host__ __device__ inline int vs_split_stride(int nwpr) { return 2 * nwpr + 1; }
+__host__ __device__ inline bool vs_split(int roi) {
+  const int nr = 2 * roi + 1;
+  return 2 * 4 * (size_t)nr * vs_split_stride((nr + 63) / 64) <= 48 * 1024;
+}
 struct VsTables {
   size_t off, step, bytes;
-  int groups;
+  int groups, chunks;
 };
 __host__ __device__ inline VsTables vs_tables(int roi) {
   VsTables t;
   t.groups = (roi - 1) / 4 + 2;
+  t.chunks = vs_split(roi) ? (2 * roi + 1 + 63) / 64 : 1;
   t.off = 0;
   t.step = (size_t)t.groups * (2 * (size_t)roi + 1) * 16;
-  t.bytes = t.step + 4 * (size_t)t.groups * sizeof(VsStep4);
+  t.bytes = t.step + 4 * (size_t)t.chunks * t.groups * sizeof(VsStep4);
   return t;
 }
 void fill_vs_tables(int roi, void *out);  // vs_tables(roi).bytes bytes

@@ -31,7 +31,6 @@ namespace {
 
 constexpr int kMaxThreads = 1024;
 constexpr int kLdsWords = 8192;          // 64 KiB bitset budget
-constexpr size_t kSplitBytes = 48 * 1024;  // budget of the two split bitsets
 
 // Where a workgroup keeps its observer's bitset while the rays run:
 //   kGlobal  straight in HBM (device atomics), for bitsets above kLdsWords
@@ -48,7 +47,7 @@ constexpr size_t kSplitBytes = 48 * 1024;  // budget of the two split bitsets
 //            written.
 enum { kGlobal = 0, kLds = 1, kSplit = 2 };
 
-__device__ __host__ inline int split_stride(int nwpr) { return 2 * nwpr + 1; }
+__device__ __host__ inline int split_stride(int nwpr) { return vs_split_stride(nwpr); }
 
 // 64x64 bit transpose across a wavefront: lane l holds row l (bit c = column
 // c); returns lane l's column l as a row, i.e. result bit c = input bit l of
@@ -183,7 +182,8 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
     const int h = along_y ? wv : wv - 2 * wy, wside = along_y ? wy : wx;
     const bool lo = h < wside;
     const int m = along_y ? my : mx;
-    const int j0 = (lo ? h : h - wside) * kWave + lane;
+    const int chunk = lo ? h : h - wside;  // the wavefront's 64 rays of its side
+    const int j0 = chunk * kWave + lane;
     const int j = j0 < m ? j0 : m >= kWave ? j0 - kWave : j0 % m;
     int dx, dy;
     if (along_y) {
@@ -240,8 +240,9 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
     auto walk = [&](auto cls_c) {
       constexpr bool ay = decltype(cls_c)::value == kAxisY;
       const int sdir = __builtin_amdgcn_readfirstlane(dir);
-      ConstStep4 *rec =
-          (ConstStep4 *)(tabs + tb.step) + (size_t)((ay ? 0 : 2) + (sdir > 0)) * tb.groups;
+      ConstStep4 *rec = (ConstStep4 *)(tabs + tb.step) +
+                        ((size_t)((ay ? 0 : 2) + (sdir > 0)) * tb.chunks +
+                         (kMode == kSplit ? chunk : 0)) * tb.groups;
       // siting.cpp:337-344 for one step.  The point slope is siting.cpp:340,
       // (float)(pel - oalt) / (float)a: with |pel - oalt| < 2^21 and
       // a < 2^14 the quotient is never closer than 2^-39 (relative) to a
@@ -252,18 +253,25 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
       // slopes are never NaN and the sign of a zero horizon changes
       // nothing downstream.)  t: the offset table's entry 2 (off + roi).
       auto step = [&](bool take, int a, float fa, double rda, uint32_t wofs, uint32_t wbit,
-                      int pel, int t) {
+                      uint64_t same, int pel, int t) {
         const float sl = (float)((double)(pel - oalt) * rda);
         horizon = take ? fmax_plain(horizon, sl) : horizon;
         const float rise = __fmul_rn(horizon, fa);
         const float halt = __fadd_rn(foalt, rise);  // siting.cpp:342
         const bool vis = take && (float)(pel + tht) >= halt;
-        if (kMode == kSplit && vis) {
-          // y-major: row off + roi, column c; x-major, transposed: column
-          // off + roi, row c; the row's byte offset (off + roi) 4 ps is t 2ps
-          __hip_atomic_fetch_or((LdsU32 *)(lds8 + wofs + __umul24(t, ps2v)), wbit,
-                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        } else if (kMode != kSplit && vis) {
+        if (kMode == kSplit) {
+          // A lane whose left neighbour is visible in the same cell leaves
+          // the OR to it (the first visible lane of each run of lanes on one
+          // cell ORs; lanes on one cell OR the same bit of the same word, so
+          // their ORs would only serialise).
+          const uint64_t v = __ballot(vis);
+          if (__builtin_amdgcn_inverse_ballot_w64(v & ~((v << 1) & same))) {
+            // y-major: row off + roi, column c; x-major, transposed: column
+            // off + roi, row c; the row's byte offset (off + roi) 4 ps is t 2ps
+            __hip_atomic_fetch_or((LdsU32 *)(lds8 + wofs + __umul24(t, ps2v)), wbit,
+                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+        } else if (vis) {
           // (24-bit products: rows, columns and strides are below 2^15)
           const int c = __builtin_amdgcn_readfirstlane(roi + sdir * a);
           const int row = ay ? t >> 1 : c, col = ay ? c : t >> 1;
@@ -317,11 +325,12 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
         asm volatile("" ::"s"(r.rd[0]), "s"(r.rd[1]), "s"(r.rd[2]), "s"(r.rd[3]),
                      "s"(r.fa[0]), "s"(r.fa[1]), "s"(r.fa[2]), "s"(r.fa[3]), "s"(r.wofs[0]),
                      "s"(r.wofs[1]), "s"(r.wofs[2]), "s"(r.wofs[3]), "s"(r.wbit[0]),
-                     "s"(r.wbit[1]), "s"(r.wbit[2]), "s"(r.wbit[3]));
+                     "s"(r.wbit[1]), "s"(r.wbit[2]), "s"(r.wbit[3]), "s"(r.same[0]),
+                     "s"(r.same[1]), "s"(r.same[2]), "s"(r.same[3]));
 #pragma unroll
         for (int u = 0; u < 4; u++)
           step(!kChecked || a + u <= last, a + u, r.fa[u], r.rd[u], r.wofs[u], r.wbit[u],
-               pel[u], (int)tc[u]);
+               r.same[u], pel[u], (int)tc[u]);
         srow += 4 * sd;
       };
       // steps every lane takes, then the rest, each lane up to its own last;
@@ -401,6 +410,18 @@ __global__ __launch_bounds__(kTile * kTileRows) void dem_transpose(
 
 }  // namespace
 
+// the minor offset of step a on the ray whose perimeter cell has minor
+// coordinate m: the kernel's ray slope is fl(minor / major), |major| = roi;
+// the offset's magnitude is fl(a |slope|) rounded half away from zero, its
+// sign that of m (on every side of the box)
+static int vs_offset(int roi, int a, int m) {
+  const float aslope = std::fabs((float)m / (float)roi);
+  const volatile float prod = (float)a * aslope;  // one rounding each
+  const volatile float biased = prod + kHalfDown;
+  const int o = (int)biased;
+  return m < 0 ? -o : o;
+}
+
 void fill_vs_tables(int roi, void *out) {
   const VsTables t = vs_tables(roi);
   uint8_t *b = (uint8_t *)out;
@@ -411,30 +432,36 @@ void fill_vs_tables(int roi, void *out) {
   for (int g = 0; g < t.groups; g++) {
     for (int u = 0; u < 4; u++) {
       const int a = 4 * g + 1 + u;
-      for (int m = -roi; m <= roi; m++) {
-        int o = 0;
-        if (a <= roi) {
-          // the kernel's ray slope is fl(minor / major), |major| = roi; the
-          // offset's magnitude is fl(a |slope|) rounded half away from zero,
-          // its sign that of the minor coordinate (on every side of the box)
-          const float aslope = std::fabs((float)m / (float)roi);
-          const volatile float prod = (float)a * aslope;  // one rounding each
-          const volatile float biased = prod + kHalfDown;
-          o = (int)biased;
-          if (m < 0) o = -o;
-        }
-        off[((size_t)g * cols + (m + roi)) * 4 + u] = (uint32_t)(2 * (o + roi));
-      }
+      const bool in = a <= roi;
+      for (int m = -roi; m <= roi; m++)
+        off[((size_t)g * cols + (m + roi)) * 4 + u] =
+            (uint32_t)(2 * ((in ? vs_offset(roi, a, m) : 0) + roi));
       for (int cls = 0; cls < 4; cls++) {
         const bool ay = cls < 2;
         const int dir = (cls & 1) ? 1 : -1;
-        VsStep4 &r = rec[(size_t)cls * t.groups + g];
-        const bool in = a <= roi;
         const int c = roi + dir * a;
-        r.rd[u] = in ? 1.0 / (double)a : 0.0;
-        r.fa[u] = (float)a;
-        r.wofs[u] = in ? 4u * (uint32_t)((ay ? 0 : nr * ps) + (c >> 5)) : 0u;
-        r.wbit[u] = in ? 1u << (c & 31) : 0u;
+        // the side's rays as the kernel deals them: chunk ch, lane l takes
+        // ray j of the side, minor coordinate m = m0 + j
+        const int mside = ay ? 2 * roi - 1 : 2 * roi + 1, m0 = ay ? -roi + 1 : -roi;
+        for (int ch = 0; ch < t.chunks; ch++) {
+          VsStep4 &r = rec[((size_t)cls * t.chunks + ch) * t.groups + g];
+          r.rd[u] = in ? 1.0 / (double)a : 0.0;
+          r.fa[u] = (float)a;
+          r.wofs[u] = in ? 4u * (uint32_t)((ay ? 0 : nr * ps) + (c >> 5)) : 0u;
+          r.wbit[u] = in ? 1u << (c & 31) : 0u;
+          uint64_t same = 0;
+          if (in && vs_split(roi)) {
+            int prev = 0;
+            for (int l = 0; l < 64; l++) {
+              const int j0 = ch * 64 + l;
+              const int j = j0 < mside ? j0 : mside >= 64 ? j0 - 64 : j0 % mside;
+              const int o = vs_offset(roi, a, m0 + j);
+              if (l > 0 && o == prev) same |= 1ull << l;
+              prev = o;
+            }
+          }
+          r.same[u] = same;
+        }
       }
     }
   }
@@ -458,7 +485,7 @@ siting_status launch_viewshed(const Geometry &g, const uint16_t *elevs, uint16_t
     if (t >= kWave && t <= kMaxThreads && t % kWave == 0) threads = t;
   }
   const size_t split_bytes = 2 * sizeof(uint32_t) * g.d.nr * (size_t)split_stride(g.d.nwpr);
-  const int mode = split_bytes <= kSplitBytes ? kSplit : words <= kLdsWords ? kLds : kGlobal;
+  const int mode = vs_split(g.p.roi) ? kSplit : words <= kLdsWords ? kLds : kGlobal;
   const size_t lds = mode == kSplit ? split_bytes
                      : mode == kLds ? sizeof(uint64_t) * words : 0;
 #define SITING_VS(A)                                                               \
