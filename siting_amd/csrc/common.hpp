@@ -119,6 +119,8 @@ siting_status launch_findmax(const Geometry &g, uint8_t *vix, int32_t *obs,
 //         step: 2 (o + roi), o = the ray's minor offset at step a,
 //         sign(m) * round-half-away(fl(a * fl(|m| / roi)))
 //         (siting.cpp:323-330); 2 roi past roi
+//   disk  per signed minor coordinate m, uint32: the last step of the ray
+//         inside the roi disk (the steps inside form a prefix)
 //   step  per ray class (major axis y or x, times direction -1 or +1),
 //         chunk of 64 rays of a side (in the split-bitset mode; one chunk
 //         otherwise) and group, a VsStep4 of the group's wavefront-uniform
@@ -141,7 +143,7 @@ __host__ __device__ inline bool vs_split(int roi) {
   return 2 * 4 * (size_t)nr * vs_split_stride((nr + 63) / 64) <= 48 * 1024;
 }
 struct VsTables {
-  size_t off, step, bytes;
+  size_t off, step, disk, bytes;
   int groups, chunks;
 };
 __host__ __device__ inline VsTables vs_tables(int roi) {
@@ -150,7 +152,8 @@ __host__ __device__ inline VsTables vs_tables(int roi) {
   t.chunks = vs_split(roi) ? (2 * roi + 1 + 63) / 64 : 1;
   t.off = 0;
   t.step = (size_t)t.groups * (2 * (size_t)roi + 1) * 16;
-  t.bytes = t.step + 4 * (size_t)t.chunks * t.groups * sizeof(VsStep4);
+  t.disk = t.step + 4 * (size_t)t.chunks * t.groups * sizeof(VsStep4);
+  t.bytes = t.disk + (2 * (size_t)roi + 1) * 4;
   return t;
 }
 void fill_vs_tables(int roi, void *out);  // vs_tables(roi).bytes bytes

@@ -154,6 +154,8 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
   const int oalt = (int)elevs[o] + oht;
   const float foalt = (float)oalt;
   const int rr = roi * roi;
+  // an observer at least roi cells from every edge of the terrain
+  const bool interior = ox >= roi && oy >= roi && ox + roi < n && oy + roi < n;
 
   if (threadIdx.x == 0) {  // observer sees itself, siting.cpp:279
     if (kMode == kSplit)
@@ -217,7 +219,10 @@ __global__ __launch_bounds__(kMaxThreads) void viewshed_kernel(
     // monotone in i).  The steps taken are therefore 1..last, with last
     // found by bisection; the walk below then needs no exit tests.
     int last = 0;
-    {
+    if (interior) {
+      // no ray leaves the terrain within roi steps: only the disk ends it
+      last = (int)((const uint32_t *)(tabs + tb.disk))[minor + roi];
+    } else {
       int hi = len + 1;  // cell(last) inside (a = 0 is the observer), cell(hi) not
       while (hi - last > 1) {
         const int mid = (last + hi) >> 1;
@@ -464,6 +469,18 @@ void fill_vs_tables(int roi, void *out) {
         }
       }
     }
+  }
+  // the last step inside the disk: a^2 + off^2 <= roi^2 holds for a prefix
+  // of the steps (both terms grow with a)
+  uint32_t *disk = (uint32_t *)(b + t.disk);
+  for (int m = -roi; m <= roi; m++) {
+    int last = 0;
+    while (last < roi) {
+      const int a = last + 1, o = vs_offset(roi, a, m);
+      if (a * a + o * o > roi * roi) break;
+      last = a;
+    }
+    disk[m + roi] = (uint32_t)last;
   }
 }
 
