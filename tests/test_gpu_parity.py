@@ -30,6 +30,7 @@ CASES = [
     (150, 64, 100, 20, 25, 60, 0, 0.8, 20000, 9),     # nwpr exactly 3 words
     (120, 260, 100, 8, 40, 9, 0, 0.7, 20000, 10),     # bitset in one LDS array
     (130, 400, 50, 8, 43, 9, 1, 0.7, 20000, 11),      # bitset in HBM
+    (240, 70, 100, 64, 12, 400, 0, 0.8, 20000, 12),   # VIX window of 32 cells
 ]
 
 
