@@ -54,12 +54,13 @@ __device__ __forceinline__ int mad24(int a, int b, int c) {
   return r;
 }
 
-// siting.cpp:157,160, evaluated in float in the reference's order:
-// ((float)(2roi) + 0.99999f) * (float)r / 2^31, truncated.
-__device__ __forceinline__ int target_offset(float span, uint32_t r) {
-  float prod = span * (float)(int)r;
-  float q = prod / 2147483648.0f;
-  return (int)q;
+// siting.cpp:157,160: (int)(fl(fl(span * (float)r) / 2^31)) with span =
+// (float)(2roi) + 0.99999f.  Dividing by 2^31 is exact, and scaling by a
+// power of two commutes with rounding for normal floats, so this is
+// (int)fl(span31 * (float)r) with span31 = span * 2^-31 (exact): one
+// multiply instead of two.
+__device__ __forceinline__ int target_offset(float span31, uint32_t r) {
+  return (int)__fmul_rn(span31, (float)(int)r);
 }
 
 template <int kHalo, bool kRecipLds>
@@ -103,7 +104,7 @@ __global__ __launch_bounds__(kThreads) void vix_kernel(
   // the DEM as a raw buffer of 2 n^2 bytes: byte offsets below 2^31
   const __amdgpu_buffer_rsrc_t dres =
       __builtin_amdgcn_make_buffer_rsrc((void *)elevs, (short)0, 2 * n * n, kBufferFlags);
-  const float span = (float)(2 * roi) + 0.99999f;
+  const float span31 = ((float)(2 * roi) + 0.99999f) * 4.656612873077393e-10f;  // * 2^-31
   const int rr = roi * roi;
   const int total = kThreads * ntests;
   // pair q = lp * ntests + k, advanced by kThreads per step
@@ -122,8 +123,8 @@ __global__ __launch_bounds__(kThreads) void vix_kernel(
       const uint32_t p = __umul24((uint32_t)px, (uint32_t)n) + (uint32_t)py;
       const uint32_t r1 = (jl[4 * k] * p + jl[4 * k + 1]) & 0x7fffffffu;
       const uint32_t r2 = (jl[4 * k + 2] * p + jl[4 * k + 3]) & 0x7fffffffu;
-      int dx = target_offset(span, r1) - roi;
-      int dy = target_offset(span, r2) - roi;
+      int dx = target_offset(span31, r1) - roi;
+      int dy = target_offset(span31, r2) - roi;
       if (__mul24(dx, dx) + __mul24(dy, dy) > rr) {  // siting.cpp:162-166
         dx = div3(dx);
         dy = div3(dy);

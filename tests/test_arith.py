@@ -192,3 +192,21 @@ def test_div3_by_float_reciprocal():
     got = np.trunc(x.astype(np.float32) * third).astype(np.int64)
     want = np.sign(x) * (np.abs(x) // 3)
     assert np.array_equal(got, want)
+
+
+def test_target_draw_with_folded_power_of_two():
+    """VIX draws a target offset as (int)(fl(fl(span * (float)r) / 2^31))
+    (siting.cpp:157,160), computed as (int)fl(span31 * (float)r) with
+    span31 = span * 2^-31: scaling by a power of two is exact and commutes
+    with rounding for normal floats.  Checked for r at the ends of its range,
+    around powers of two and at random, for small, common and maximal roi."""
+    rng = np.random.default_rng(3)
+    two31 = np.float32(2.0 ** -31)
+    edge = np.array([0, 1, 2, 3, (1 << 31) - 1, (1 << 31) - 2] +
+                    [(1 << k) + d for k in range(1, 31) for d in (-1, 0, 1)], np.int64)
+    for roi in (1, 2, 25, 50, 100, 150, 333, 4096, 16383):
+        span = np.float32(np.float32(2 * roi) + np.float32(0.99999))
+        r = np.concatenate([edge, rng.integers(0, 1 << 31, 400_000)]).astype(np.float32)
+        ref = (np.float32(span * r) / np.float32(2147483648.0)).astype(np.int64)
+        mine = (np.float32(span * two31) * r).astype(np.int64)
+        assert np.array_equal(ref, mine), roi
