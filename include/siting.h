@@ -92,8 +92,8 @@ typedef struct {
  * library call of the reference's stage functions does).  Returns
  * SITING_ESMALLBLOCK where the reference dies with "last block too small",
  * SITING_EINVAL for nonsensical values (nrows, roi, ntests < 1, bw < 1,
- * intervis not 0/1, nrows/bw rounding to zero blocks, sizes whose arrays
- * overflow 32-bit indexing). */
+ * roi > 10000 (the CLI's bound), intervis not 0/1, nrows/bw rounding to
+ * zero blocks, sizes whose arrays overflow 32-bit indexing). */
 siting_status siting_derive(const siting_params *p, siting_derived *d);
 
 /* The reference CLI's range checks (siting.cpp:680-686), in its order.

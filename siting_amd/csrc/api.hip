@@ -32,7 +32,7 @@ siting_status site_loop_failed() {
 siting_status derive(const siting_params *p, siting_derived *d) {
   if (!p || !d) return SITING_EINVAL;
   const int n = p->nrows;
-  if (n <= 0 || n > 32767 || p->roi < 1 || p->roi > 16383 || p->ntests < 1 ||
+  if (n <= 0 || n > 32767 || p->roi < 1 || p->roi > kMaxRoi || p->ntests < 1 ||
       p->ntests > 4000 || p->bw < 1 || p->nwanted0 < 1 || p->oht < 0 ||
       p->tht < 0 || p->oht > 1000000 || p->tht > 1000000 ||
       (p->intervis != 0 && p->intervis != 1))
@@ -441,7 +441,7 @@ siting_status dalloc(DevBuf &b, size_t bytes) {
 
 siting_status siting_calc_vix(int nrows, const uint16_t *elevs, int roi, int oht,
                               int tht, int ntests, uint8_t *vix) {
-  if (!elevs || !vix || nrows <= 0 || nrows > 32767 || roi < 1 || roi > 16383 ||
+  if (!elevs || !vix || nrows <= 0 || nrows > 32767 || roi < 1 || roi > kMaxRoi ||
       ntests < 1 || ntests > 4000)
     return SITING_EINVAL;
   hipStream_t s;
@@ -504,7 +504,7 @@ siting_status siting_calc_sheds(int nrows, const uint16_t *elevs, int roi, int o
                                 int tht, int nsheds, const int32_t *obs,
                                 uint64_t *sheds) {
   if (!elevs || !obs || !sheds || nrows <= 0 || nrows > 32767 || roi < 1 ||
-      roi > 16383 || nsheds < 0)
+      roi > kMaxRoi || nsheds < 0)
     return SITING_EINVAL;
   const size_t n2 = (size_t)nrows * nrows;
   for (int k = 0; k < nsheds; k++)
@@ -544,7 +544,7 @@ siting_status siting_site_it(int nrows, int roi, int intervis, int nsheds,
                              const int32_t *obs, const uint64_t *sheds,
                              uint8_t *selected, siting_stats *stats) {
   if (!obs || !sheds || !selected || nrows <= 0 || nrows > 32767 || roi < 1 ||
-      roi > 16383 || nsheds < 1 || (intervis != 0 && intervis != 1))
+      roi > kMaxRoi || nsheds < 1 || (intervis != 0 && intervis != 1))
     return SITING_EINVAL;
   const size_t n2 = (size_t)nrows * nrows;
   for (int k = 0; k < nsheds; k++)

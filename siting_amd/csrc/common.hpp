@@ -58,6 +58,11 @@ __host__ __device__ inline void col_split(int c, int &word, int &bit) {
 // zero) is trunc(fl(x + kHalfDown)): checked for every such float.
 constexpr float kHalfDown = 0.49999997f;
 
+// The largest roi the library accepts: the reference CLI's bound
+// (siting.cpp:680-686).  It keeps VIEWSHED's per-roi tables (about
+// 0.8 GB at the bound) within the 32-bit offsets of buffer loads.
+constexpr int kMaxRoi = 10000;
+
 // ---------------------------------------------------------------------------
 // wave / block reductions
 // ---------------------------------------------------------------------------

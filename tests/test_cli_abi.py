@@ -135,3 +135,13 @@ def test_dem_round_trip_and_short_file(sa, tmp_path):
     assert np.array_equal(sa.dem_read(path, 37), dem)
     with pytest.raises(sa.SitingError):
         sa.dem_read(path, 38)
+
+
+def test_library_roi_bound(sa):
+    """The library accepts roi up to the CLI's bound of 10000
+    (siting.cpp:680-686) and rejects larger ones with SITING_EINVAL."""
+    d = sa.derive(sa.make_params(3000, 10000, 100, 64, 16, 10000, 0))
+    assert d.nr == 2 * 10000 + 1
+    with pytest.raises(sa.SitingError) as err:
+        sa.derive(sa.make_params(3000, 10001, 100, 64, 16, 10000, 0))
+    assert err.value.status == sa.EINVAL
