@@ -471,14 +471,13 @@ void fill_vs_tables(int roi, void *out) {
     }
   }
   // the last step inside the disk: a^2 + off^2 <= roi^2 holds for a prefix
-  // of the steps (both terms grow with a)
+  // of the steps (both terms grow with a), found by bisection
   uint32_t *disk = (uint32_t *)(b + t.disk);
   for (int m = -roi; m <= roi; m++) {
-    int last = 0;
-    while (last < roi) {
-      const int a = last + 1, o = vs_offset(roi, a, m);
-      if (a * a + o * o > roi * roi) break;
-      last = a;
+    int last = 0, hi = roi + 1;  // step last inside (0: the observer), hi not
+    while (hi - last > 1) {
+      const int a = (last + hi) >> 1, o = vs_offset(roi, a, m);
+      if (a * a + o * o <= roi * roi) last = a; else hi = a;
     }
     disk[m + roi] = (uint32_t)last;
   }
