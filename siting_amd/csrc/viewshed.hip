@@ -3,18 +3,25 @@
 // arithmetic).
 //
 // A workgroup owns one observer.  Its lanes take the 8*roi rays to the cells
-// of the box perimeter (each corner once, siting.cpp:292-304); adjacent lanes
-// take adjacent perimeter cells, so at every step the wavefront reads a
-// short arc of neighbouring DEM cells.  Along a ray the running horizon
-// slope is kept in float exactly as the reference computes it: the IEEE
-// quotient for the ray slope and the point slope (the latter via an exact
-// double reciprocal, see below), round-half-away for the minor coordinate,
-// no fused multiply-add (siting.cpp:323-344).  Visible
-// cells are OR'ed into the shed bitset held in LDS (MSB-first 64-bit words,
-// siting.cpp:247-250); the finished bitset goes to HBM with coalesced
-// 64-bit stores, zero words included, so no separate clear is needed.  When
-// the bitset does not fit the LDS budget (roi above ~350) the rays OR
-// straight into HBM with device atomics after the workgroup clears it.
+// of the box perimeter (each corner once, siting.cpp:292-304), dealt by side
+// of the box in whole wavefronts, so that a wavefront's rays share major
+// axis and direction and adjacent lanes take adjacent perimeter cells: at
+// every step the wavefront reads a short arc of one DEM row (y-major rays
+// read a transposed copy), and the step's row, reciprocal and bit position
+// are wavefront-uniform.  Along a ray the running horizon slope is kept in
+// float exactly as the reference computes it: the IEEE quotient for the ray
+// slope and the point slope (the latter via an exact double reciprocal, see
+// below), round-half-away for the minor coordinate, no fused multiply-add
+// (siting.cpp:323-344).  Everything a step needs that depends only on roi
+// comes from per-roi tables filled on the host (vs_tables): the minor offsets
+// of every ray, the reciprocals, the bit positions, the lanes that share a
+// cell, and each ray's last step inside the roi disk.  Visible cells are
+// OR'ed into bitsets in LDS: up to roi ~200 two of 32-bit words (one
+// transposed for the x-major rays), larger ones in the output layout (the
+// MSB-first 64-bit words of siting.cpp:247-250), and beyond the LDS budget
+// (roi above ~350) straight into HBM with device atomics.  The finished
+// bitset goes to HBM with coalesced 64-bit stores, zero words included, so
+// no separate clear is needed.
 //
 // Observers are assigned XCD-major (workgroup b runs on XCD b%8, each XCD
 // takes a contiguous run of observers, which FINDMAX emitted block by block,
