@@ -929,7 +929,9 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         my_u += v < 0 ? -v : v;
       }
       __syncthreads();  // counts read before the areas overwrite them
-      if (!stop && lcnt > 0) union_wait();
+      // (an intervis update reads cum | the winner's bits, the same before,
+      // during and after the union: it needs no wait)
+      if (!stop && lcnt > 0 && !a.intervis) union_wait();
       if (!stop && lcnt > 0 && !failed) {
         // one wavefront per update candidate
         for (int e = wid; e < lcnt; e += kLoopWaves) {
@@ -957,7 +959,9 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     }
     for (int i = tid; i < (cnt + 31) / 32; i += kLoopThreads) seen[i] = 0;
     if (stop) break;
-    union_wait();  // before the re-evaluation reads cum
+    // before the re-evaluation reads cum (intervis rounds re-evaluate
+    // nothing; the end-of-round barrier still counts every union)
+    if (!a.intervis) union_wait();
     if (failed) {
       niter = -1;
       break;
