@@ -121,7 +121,13 @@ def params_of(sa, w):
 
 
 def algorithmic_bytes(n, d, st):
-    """Algorithmic HBM bytes per stage for one job (formulas in DESIGN.md)."""
+    """Algorithmic HBM bytes per stage for one job: the bytes the reference's
+    sequential algorithm moves.  VIX reads the DEM and writes vix; FINDMAX
+    scans vix once per observer wanted per block (nwpb passes) and writes
+    obs; VIEWSHED writes every shed and reads the DEM; SITE reads one shed
+    per union and per update-list re-evaluation (sum_u + nused of them) and
+    scans area (4 B) and used (1 B) of every candidate in each iteration
+    (siting.cpp:501 site_it)."""
     S = 8 * d.shed_words
     nsheds = d.nwanted
     return {
