@@ -127,7 +127,7 @@ def algorithmic_bytes(n, d, st):
     obs; VIEWSHED writes every shed and reads the DEM; SITE reads one shed
     per union and per update-list re-evaluation (sum_u + nused of them) and
     scans area (4 B) and used (1 B) of every candidate in each iteration
-    (siting.cpp:501 site_it)."""
+    (siting.cpp:540-562, the loop of site_it)."""
     S = 8 * d.shed_words
     nsheds = d.nwanted
     return {
