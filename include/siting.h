@@ -254,7 +254,10 @@ typedef struct {
 
 siting_status siting_server_create(int device, const siting_server_opts *opts,
                                    siting_server **out);
-/* Drops queued requests (FAILED, SITING_EBUSY), waits for running ones. */
+/* Drops queued requests (FAILED, SITING_EBUSY), waits for running ones,
+ * then for every thread already blocked in siting_server_wait to return
+ * (they see their request DONE or FAILED).  No other call on s may start
+ * once destroy has begun. */
 void siting_server_destroy(siting_server *s);
 /* Copies the DEM.  Rejects malformed parameters (SITING_EINVAL /
  * SITING_ESMALLBLOCK) and jobs larger than the whole budget

@@ -309,6 +309,12 @@ class Job:
         _check(lib().siting_job_download(self._h, which, _ptr(out), out.nbytes))
         return out
 
+    def download_into(self, which, out: np.ndarray):
+        """Copies the first out.nbytes bytes of buffer `which` into out."""
+        if not out.flags.c_contiguous:
+            raise SitingError(EINVAL, "output array must be contiguous")
+        _check(lib().siting_job_download(self._h, which, _ptr(out), out.nbytes))
+
     def download_vix(self):
         return self._download(BUF_VIX, np.uint8, self.params.nrows ** 2)
 

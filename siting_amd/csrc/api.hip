@@ -21,10 +21,13 @@ siting_status hip_fail(hipError_t e, const char *what, const char *file, int lin
   return SITING_EDEVICE;
 }
 
-// The SITE loop gave up at a grid barrier that did not complete (niter < 0).
-siting_status site_loop_failed() {
-  std::snprintf(g_last_error, sizeof g_last_error,
-                "SITE loop: a grid barrier timed out (workgroups not co-resident)");
+// The SITE loop gave up (niter < 0): -1 when a grid barrier did not
+// complete, -2 when its workgroups disagreed on a round's picks.
+siting_status site_loop_failed(int niter) {
+  std::snprintf(g_last_error, sizeof g_last_error, "%s",
+                niter == -2 ? "SITE loop: workgroups disagreed on a round's picks"
+                            : "SITE loop: a grid barrier timed out (workgroups not "
+                              "co-resident)");
   return SITING_EDEVICE;
 }
 
@@ -383,7 +386,7 @@ siting_status siting_job_stats(siting_job *j, siting_stats *st) {
     st->sum_u = (int64_t)s.sum_u;
     const double n2 = (double)j->g.p.nrows * (double)j->g.p.nrows;
     st->coverage = 100.0 * (double)s.cumarea / n2;
-    if (s.niter < 0) return site_loop_failed();
+    if (s.niter < 0) return site_loop_failed(s.niter);
   }
   return SITING_OK;
 }
@@ -591,7 +594,7 @@ siting_status siting_site_it(int nrows, int roi, int intervis, int nsheds,
     stats->sum_u = (int64_t)st.sum_u;
     stats->coverage = 100.0 * (double)st.cumarea / (double)n2;
   }
-  if (st.niter < 0) return site_loop_failed();
+  if (st.niter < 0) return site_loop_failed(st.niter);
   return SITING_OK;
 }
 

@@ -199,6 +199,7 @@ struct SiteArgs {
   uint8_t *sused;    // [nsheds] used state of workgroups in global mode
   uint64_t *nb;      // [kSiteMaxBatch][nr][nwpr + 1] bits each pick newly covered
   unsigned int *sseen;  // [G][nsheds/32 + 1] seen bitmaps of global-mode groups
+  uint64_t *phash;   // [2][kMaxGroups] per-round hash of each workgroup's picks
   int n, roi, nr, nwpr, cw, nsheds, intervis;
   int cs, ncr, Q, nunits, G, upw, kpu, nbuckets, cap;
   // dynamic LDS layout (byte offsets)
@@ -211,6 +212,8 @@ struct SiteArgs {
                   // decremented (0: never)
   long long stop_area;  // least cumulative area c with 100.0 * c / (n * n) > 95
   unsigned int spin_limit;  // barrier polls before the loop gives up
+  int hash_fault;  // test knob: round whose pick hash the last workgroup
+                   // alters (0: none), to exercise the divergence check
 };
 
 // Device scratch SITE needs beyond SiteBuffers' arrays, and the size of the

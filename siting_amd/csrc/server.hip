@@ -111,6 +111,7 @@ struct siting_server {
   std::vector<Request *> active;
   uint64_t next_id = 1;
   int groups_in_use = 0;
+  int waiters = 0;  // threads blocked in siting_server_wait
   bool stop = false;
   std::thread thread;
   Clock::time_point t0;
@@ -303,16 +304,25 @@ void siting_server_destroy(siting_server *S) {
     for (Request *r : S->pending) {
       r->state = SITING_JOB_FAILED;
       r->status = SITING_EBUSY;
+      r->t_done = S->now();
     }
     S->pending.clear();
+    S->cv_done.notify_all();  // waiters on dropped requests return now
   }
   S->cv_work.notify_all();
+  // the scheduler retires the running jobs (waking their waiters) and exits
   if (S->thread.joinable()) S->thread.join();
+  {
+    // every request is now DONE or FAILED: let the blocked waiters leave
+    // before their requests and the server go away
+    std::unique_lock<std::mutex> lk(S->mu);
+    S->cv_done.notify_all();
+    S->cv_done.wait(lk, [&] { return S->waiters == 0; });
+  }
   (void)hipSetDevice(S->device);
   for (hipStream_t s : S->streams) (void)hipStreamDestroy(s);
   if (S->block) (void)hipFree(S->block);
   for (auto &kv : S->reqs) delete kv.second;
-  S->cv_done.notify_all();
   delete S;
 }
 
@@ -377,12 +387,16 @@ siting_status siting_server_wait(siting_server *S, uint64_t id, double timeout_s
   if (it == S->reqs.end()) return SITING_ENOTFOUND;
   Request *r = it->second;
   auto pred = [&] { return r->state == SITING_JOB_DONE || r->state == SITING_JOB_FAILED; };
+  S->waiters++;
+  bool done = true;
   if (timeout_s < 0)
     S->cv_done.wait(lk, pred);
-  else if (!S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred))
-    return SITING_EBUSY;
-  if (info) fill_info(r, info);
-  return SITING_OK;
+  else
+    done = S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred);
+  if (done && info) fill_info(r, info);
+  // a destroy in progress waits for the last waiter before freeing anything
+  if (--S->waiters == 0 && S->stop) S->cv_done.notify_all();
+  return done ? SITING_OK : SITING_EBUSY;
 }
 
 siting_status siting_server_result(siting_server *S, uint64_t id, uint8_t *selected,

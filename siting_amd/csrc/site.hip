@@ -213,10 +213,7 @@ __global__ __launch_bounds__(256) void site_first_pass(SiteArgs a) {
 
 // ---- persistent loop -----------------------------------------------------
 
-// Grid barrier: release on arrival, relaxed spin, one acquire fence after.
-// (An acquire load inside the spin would invalidate the cache on every poll,
-// and the polling groups would keep evicting the working set of the groups
-// still computing.)
+// Grid barrier without fences.
 // Data the workgroups of a loop exchange (the published unit keys, the bits
 // each pick newly covered, the union cum) is only ever read and written with
 // device-coherent accesses: relaxed agent-scope atomic loads and stores
@@ -224,7 +221,15 @@ __global__ __launch_bounds__(256) void site_first_pass(SiteArgs a) {
 // write-back or invalidation.  The grid barrier therefore only has to wait
 // for each wavefront's stores to complete before arriving, and uses no
 // agent-scope fence (on gfx950 those write back and invalidate the whole of
-// the XCD's L2, for every workgroup of every loop running).
+// the XCD's L2, for every workgroup of every loop running).  A relaxed
+// barrier is sound only because no exchanged word is ever read with a plain
+// load: a plain load may hit a stale line in this CU's L1 or another XCD's
+// L2, and workgroups that read different keys pick differently.  (An early
+// version read the published keys with plain loads under a relaxed barrier
+// and hung: the workgroups' picks diverged.)  The loop also compares a hash
+// of every workgroup's picks each round, so a divergence that kept the
+// barrier counts equal would end the loop with an error instead of a wrong
+// selection.
 template <class T>
 __device__ __forceinline__ T ld_dev(const T *p) {
   return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -608,8 +613,30 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
 
   int nused = 0, niter = started ? 1 : -1, parity = 0;
   long long cumarea = 0, my_u = 0;
+  // Divergence check: every workgroup computes the round's picks from the
+  // same published keys, so all must agree.  The last wavefront publishes a
+  // hash of this workgroup's picks each round and, during the next round's
+  // key gather (which it takes no part in), compares every workgroup's hash
+  // with its own.  All workgroups read the same G values, so either all
+  // continue or all leave the loop at the same point (niter = -2), without
+  // a barrier that some would never reach.
+  // (kept in LDS: registers are the loop's scarcest resource)
+  __shared__ int ph_bad, ph_round;
+  __shared__ uint64_t ph_prev;  // this group's hash of the previous round
+  if (tid == 0) ph_round = 0;
+  __syncthreads();
 
   while (started) {
+    if (wid == kLoopWaves - 1) {
+      bool bad = false;
+      if (ph_round > 0) {
+        const uint64_t *ph = a.phash + (size_t)(parity ^ 1) * kMaxGroups;
+        const uint64_t mine = ph_prev;
+        for (int w = lane; w < G; w += kWave) bad |= ld_dev(&ph[w]) != mine;
+      }
+      const bool any_bad = __any(bad);
+      if (lane == 0) ph_bad = any_bad;
+    }
     // ---- pick the round's winners (identical in every workgroup).  Winners
     // are a prefix of the descending order of all published unit keys in
     // which no two are within 2*roi of each other: none of them is in
@@ -702,10 +729,36 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       }
     }
     __syncthreads();
+    if (ph_bad) {
+      niter = -2;
+      break;
+    }
     int k = bw_n;
     // no pick while the largest key is stale: the round only re-evaluates
     const bool eval_only = k == 0 && bw_head > 0;
     if (k == 0 && !eval_only) break;  // cpp:589-592
+    if (wid == kLoopWaves - 1) {
+      // hash of (picks, count, head); stored before this round's barriers
+      uint64_t v = 0;
+      if (lane < k) {
+        v = bw_key[lane] ^ ((uint64_t)bw_pos[lane] * 0x9E3779B97F4A7C15ull) ^
+            ((uint64_t)(lane + 1) << 58);
+        v ^= v >> 31;
+        v *= 0xBF58476D1CE4E5B9ull;
+        v ^= v >> 29;
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v ^= __shfl_xor(v, o, kWave);
+      v ^= ((uint64_t)(uint32_t)k << 32) | (uint32_t)bw_head;
+      const int round = ph_round + 1;
+      // test knob: the last workgroup's hash of round hash_fault is altered
+      if (a.hash_fault == round && g == G - 1) v ^= 1;
+      if (lane == 0) {
+        ph_prev = v;
+        ph_round = round;
+        st_dev(&a.phash[(size_t)parity * kMaxGroups + g], v);
+      }
+    }
     // update candidates below tau become stale; stale ones reaching tau are
     // re-evaluated (areas only shrink, so a bound never falls below the area)
     const int tau = hybrid ? bw_head - (bw_head >> a.tau_shift) : 0;
@@ -1217,6 +1270,7 @@ struct Scratch {
   uint32_t *kpos;
   uint64_t *nb;
   unsigned int *sseen;
+  uint64_t *phash;
 };
 
 size_t scratch_layout(const Plan &p, int nsheds, uint8_t *base, Scratch *out) {
@@ -1242,6 +1296,7 @@ size_t scratch_layout(const Plan &p, int nsheds, uint8_t *base, Scratch *out) {
   s.sseen = (unsigned int *)take(4 * ((size_t)nsheds / 32 + 2) * kMaxGroups);
   s.keys = (uint64_t *)take(8 * 4 * (size_t)p.nunits);
   s.kpos = (uint32_t *)take(4 * 4 * (size_t)p.nunits);
+  s.phash = (uint64_t *)take(8 * 2 * (size_t)kMaxGroups);
   if (out) *out = s;
   return o;
 }
@@ -1283,6 +1338,7 @@ siting_status site_make_args(const Geometry &g, const int32_t *obs,
   a.sused = sc.sused;
   a.nb = sc.nb;
   a.sseen = sc.sseen;
+  a.phash = sc.phash;
   a.n = n;
   a.roi = roi;
   a.nr = g.d.nr;
@@ -1317,6 +1373,8 @@ siting_status site_make_args(const Geometry &g, const int32_t *obs,
   // exiting normally)
   const char *sl = std::getenv("SITING_SITE_SPIN_LIMIT");
   a.spin_limit = sl && *sl && std::atoi(sl) >= 0 ? (unsigned)std::atoi(sl) : kSpinLimit;
+  const char *hf = std::getenv("SITING_SITE_HASH_FAULT");
+  a.hash_fault = hf && *hf && std::atoi(hf) > 0 ? std::atoi(hf) : 0;
   *out = a;
   if (lds) *lds = p.lds;
   return SITING_OK;

@@ -148,3 +148,28 @@ def test_cli_drop_in(gpu, orc, tmp_path):
     assert keys == ["input", "vix", "findmax", "viewshed", "nusedsheds", "coverage",
                     "site", "output", "total"]
     assert f" nusedsheds:{st.nused} coverage:{st.coverage:g} " in line
+
+
+@pytest.mark.parametrize("iv", [0, 1])
+def test_site_global_candidate_mode(gpu, orc, monkeypatch, iv):
+    """SITE's global-memory candidate state: a loop workgroup keeps its
+    candidates' ids, positions, areas and flags in LDS when they fit (about
+    9600 with the loop's 132 KiB), else in a global slice of the same layout.
+    One workgroup (the SITING_SITE_GROUPS tuning cap) over 19044 candidates
+    (1100^2, bw 8) takes the global slice."""
+    n, roi, ht, nt, bw, nw0 = 1100, 30, 100, 32, 8, 19044
+    print(f"[case] SITE global candidate mode iv={iv}", flush=True)
+    dem = make_dem(gpu, n, 0.7, 20000, 71 + iv)
+    ref = orc.run(n, dem, roi, ht, nt, bw, nw0, iv, nthreads=8)
+    assert ref["derived"].nwanted == 19044
+    monkeypatch.setenv("SITING_SITE_GROUPS", "1")
+    with gpu.Job(gpu.make_params(n, roi, ht, nt, bw, nw0, iv)) as job:
+        job.upload_dem(dem)
+        job.run()
+        job.sync()
+        st = job.stats()
+        np.testing.assert_array_equal(job.download_selected(), ref["selected"])
+        np.testing.assert_array_equal(job.download_order(st.nused), ref["order"])
+        rs = ref["stats"]
+        assert (st.nused, st.niter, st.cumarea, st.sum_u) == \
+            (rs.nused, rs.niter, rs.cumarea, rs.sum_u)

@@ -77,3 +77,33 @@ def test_server_rejects_bad_requests(gpu, mix):
         assert info.state == 2
         sel, _ = srv.result(jid)
         np.testing.assert_array_equal(sel, ref["selected"])
+
+
+def test_server_destroy_wakes_blocked_waiters(gpu, mix):
+    """Destroying the server while other threads are blocked waiting on its
+    requests: each waiter returns with its request DONE or FAILED (dropped
+    from the queue), and destroy returns only after they have left."""
+    import threading
+    print("[case] server destroy with blocked waiters", flush=True)
+    import time
+    srv = gpu.Server(max_active=1, site_groups=8)
+    ids = [srv.submit(p, dem) for _ in range(4) for (p, dem, _) in mix]
+    states, errors = {}, []
+
+    def waiter(jid):
+        try:
+            states[jid] = srv.wait(jid, 60.0).state
+        except Exception as e:  # noqa: BLE001
+            errors.append(e)
+
+    ths = [threading.Thread(target=waiter, args=(j,)) for j in ids]
+    for t in ths:
+        t.start()
+    time.sleep(0.05)  # every waiter is blocked in wait; most jobs still queued
+    srv.close()
+    for t in ths:
+        t.join(30.0)
+        assert not t.is_alive()
+    assert not errors, errors
+    assert sorted(states) == sorted(ids)
+    assert all(s in (2, 3) for s in states.values())  # DONE or FAILED
