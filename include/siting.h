@@ -128,6 +128,34 @@ siting_status siting_site_it(int nrows, int roi, int intervis, int nsheds,
                              const int32_t *obs, const uint64_t *sheds,
                              uint8_t *selected, siting_stats *stats);
 
+/* site_it on device arrays, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory or create
+ * streams -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation.  It runs the same kernels as siting_site_it with
+ * the same results, taking its working memory (candidate areas, the
+ * cumulative bitset, the loop's state and scratch) from a fixed 64 MiB
+ * device array the library holds on every device it runs on until the
+ * process ends; a call that needs more returns SITING_ENOMEM, and
+ * siting_status_str then names the size it needed.  A call uses about
+ * 31-70 bytes per candidate, nrows^2/8 bytes and 256 (2 roi + 1)
+ * (ceil((2 roi + 1)/64) + 1) bytes (1.4 M candidates on 8192^2 at roi 25
+ * fit; so do 262144 on 8192^2 at roi 200).
+ *   obs, sheds  device arrays of the current device (layouts above); obs
+ *               entries must lie in [0, nrows^2), as FINDMAX writes them
+ *               (they are not checked)
+ *   selected    device uint8[nsheds], the output mask
+ *   order       device int32[nsheds] receiving the selection order, or NULL
+ *   stats       NULL: the call only enqueues work on `stream` and returns;
+ *               else it waits for the stream and fills the counters
+ *   stream      a hipStream_t of the current device, or NULL (null stream)
+ * All calls on one device share the one array: calls that overlap on a
+ * device (other streams, other threads) race and must be serialised by the
+ * caller.  With stats NULL the call can be captured into a HIP graph. */
+siting_status siting_site_it_noalloc(int nrows, int roi, int intervis, int nsheds,
+                                     const int32_t *obs, const uint64_t *sheds,
+                                     uint8_t *selected, int32_t *order,
+                                     siting_stats *stats, void *stream);
+
 /* ---- device-resident jobs --------------------------------------------
  * A job owns its device buffers (DEM, vix, obs, sheds, SITE state), so the
  * four stages run back to back in HBM with no host round trip.  All work of

@@ -111,6 +111,8 @@ _SIGS = {
     "siting_dem_generate": (_I, [_I, ctypes.c_double, _I, _I, _I, ctypes.c_double,
                                  ctypes.c_double, ctypes.c_uint64, _VP]),
     "siting_device_count": (_I, []),
+    "siting_site_it_noalloc": (_I, [_I, _I, _I, _I, _VP, _VP, _VP, _VP,
+                                    ctypes.POINTER(Stats), _VP]),
     "siting_batch_create": (_I, [_I, _I, _VP, _VP, ctypes.POINTER(_VP)]),
     "siting_batch_destroy": (None, [_VP]),
     "siting_batch_upload_dem": (_I, [_VP, _I, _VP]),

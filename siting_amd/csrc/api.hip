@@ -13,8 +13,17 @@
 namespace siting {
 
 static thread_local char g_last_error[256];
+// Set only when a call on this thread rejected a size beyond a fixed working
+// memory (siting_site_it_noalloc); cleared by its next call and by any HIP
+// failure, so that SITING_ENOMEM's message names the limit that was hit.
+static thread_local char g_limit_error[256];
+
+void set_limit_error(const char *msg) {
+  std::snprintf(g_limit_error, sizeof g_limit_error, "%s", msg ? msg : "");
+}
 
 siting_status hip_fail(hipError_t e, const char *what, const char *file, int line) {
+  g_limit_error[0] = 0;
   std::snprintf(g_last_error, sizeof g_last_error, "%s failed: %s (%s:%d)", what,
                 hipGetErrorString(e), file, line);
   if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return SITING_ENOMEM;
@@ -127,7 +136,8 @@ const char *siting_status_str(siting_status s) {
     case SITING_EINVAL: return "invalid argument";
     case SITING_ERANGE: return "parameter out of range";
     case SITING_ESMALLBLOCK: return "The last block is too small for nwantedperblock.";
-    case SITING_ENOMEM: return "Memory exhausted. Program terminates.";
+    case SITING_ENOMEM:
+      return g_limit_error[0] ? g_limit_error : "Memory exhausted. Program terminates.";
     case SITING_EDEVICE: return g_last_error[0] ? g_last_error : "HIP device error";
     case SITING_EIO: return "Input failed";
     case SITING_EBUSY: return "busy";

@@ -26,6 +26,12 @@ constexpr int kWave = 64;
   } while (0)
 
 siting_status hip_fail(hipError_t e, const char *what, const char *file, int line);
+// The SITE loop gave up (niter < 0): -1 a grid barrier timed out, -2 its
+// workgroups disagreed on a round's picks.  Sets the message, returns
+// SITING_EDEVICE.
+siting_status site_loop_failed(int niter);
+// Message of the next SITING_ENOMEM on this thread (nullptr or "" clears it).
+void set_limit_error(const char *msg);
 
 // ---------------------------------------------------------------------------
 // integer helpers shared by host and device
