@@ -55,8 +55,11 @@ def _wave_rays(roi):
     """The rays of VIEWSHED's wavefronts, as (g, lane, dx, dy, valid): each
     side of the box is dealt in whole wavefronts of 64 lanes, the rows
     dy = -roi and dy = roi without corners (y-major), then the columns
-    dx = -roi and dx = roi with them (x-major); lanes past the end of a side
-    repeat its last ray with valid False."""
+    dx = -roi and dx = roi with them (x-major).  A lane past the end of its
+    side (valid False) repeats the ray 64 places back when the side has at
+    least 64 rays, else ray j0 mod m of its own wavefront -- the kernel's
+    rule (viewshed.hip, the lane-to-ray map of the walk and of the table
+    builder)."""
     my, mx = 2 * roi - 1, 2 * roi + 1
     wy, wx = -(-my // 64), -(-mx // 64)
     for g in range(2 * (wy + wx)):
@@ -67,7 +70,7 @@ def _wave_rays(roi):
         m = my if along_y else mx
         for lane in range(64):
             j0 = (h if lo else h - wside) * 64 + lane
-            j = min(j0, m - 1)
+            j = j0 if j0 < m else (j0 - 64 if m >= 64 else j0 % m)
             if along_y:
                 dx, dy = -roi + 1 + j, (-roi if lo else roi)
             else:
@@ -132,16 +135,28 @@ def test_ray_set_is_the_box_perimeter():
     """The kernel's valid lanes cover every perimeter cell of the (2roi+1)^2
     box exactly once, as the reference's four sides do (siting.cpp:292-304);
     every wavefront walks one major axis in one direction; a padding lane
-    repeats a valid ray of its own wavefront."""
+    repeats a valid ray of its own side -- one that a different wavefront
+    walks when the side has at least 64 rays (so a duplicate never ORs the
+    same bit as its original in one instruction), else one of its own."""
     for roi in (1, 2, 3, 25, 31, 32, 33, 50, 100, 150, 333):
         waves = {}
+        side_rays = {}
         for g, lane, dx, dy, ok in _wave_rays(roi):
             along_y, length, dirn, _ = _ray(roi, dx, dy)
             waves.setdefault(g, []).append(((along_y, dirn), (dx, dy), ok))
+            if ok:
+                side_rays.setdefault((along_y, dirn), set()).add((dx, dy))
         for lanes in waves.values():
             assert len(lanes) == 64 and len({c for c, _, _ in lanes}) == 1
             assert lanes[0][2]
-            assert {r for _, r, ok in lanes if not ok} <= {r for _, r, ok in lanes if ok}
+            cls = lanes[0][0]
+            own = {r for _, r, ok in lanes if ok}
+            pad = {r for _, r, ok in lanes if not ok}
+            assert pad <= side_rays[cls]
+            if len(side_rays[cls]) >= 64:
+                assert not (pad & own)
+            else:
+                assert pad <= own
         cells = []
         for along_y, length, dirn, slope in _ray_cells(roi):
             end = _offsets(dirn, slope, np.array([length]))[0]
