@@ -84,7 +84,24 @@ typedef struct {
   int64_t sum_u;    /* sum over iterations of |update list| (cpp:540-544) */
   double coverage;  /* 100.0*cumarea/nrows^2 (" coverage:C", cpp:617) */
   float ms_vix, ms_findmax, ms_viewshed, ms_site; /* HIP event times */
+  int rounds;       /* rounds of the device's greedy loop: each takes one or
+                       more picks (at most one under intervis) */
 } siting_stats;
+
+/* Kernel shape choices of one job.  Any choice gives the same bits; only
+ * the speed differs.  0 in a field means the built-in rule. */
+typedef struct {
+  int vix_halo;          /* VIX: cells of DEM halo around a 16x16 tile kept
+                            in LDS: 8, 16 or 32 */
+  int viewshed_threads;  /* VIEWSHED: threads per observer, a multiple of 64
+                            in 64..1024 */
+  int viewshed_mode;     /* VIEWSHED bitsets: 1 two LDS bitsets (row-major and
+                            transposed), 2 one LDS bitset, 3 in HBM; a mode
+                            whose bitsets do not fit falls back to the rule */
+  int site_q;            /* SITE: column classes per cell row, 2 or 3 */
+  int site_tau_shift;    /* SITE: update candidates below head - head/2^s
+                            become stale bounds, s in 1..20 */
+} siting_tuning;
 
 /* ---- parameters ------------------------------------------------------- */
 
@@ -199,6 +216,11 @@ siting_status siting_job_download(siting_job *job, siting_buffer which,
 /* Device address of a job buffer (for callers that keep data in HBM). */
 siting_status siting_job_device_ptr(siting_job *job, siting_buffer which,
                                     void **dptr, size_t *bytes);
+/* Kernel shape choices for the job's later stage launches (NULL: the
+ * built-in rules).  A job starts with the tuning cache's entry for its
+ * parameters, when there is one (siting_tuning_lookup). */
+siting_status siting_job_set_tuning(siting_job *job, const siting_tuning *t);
+siting_status siting_job_get_tuning(siting_job *job, siting_tuning *t);
 /* Counters + per-stage device times of the last run (after sync). */
 siting_status siting_job_stats(siting_job *job, siting_stats *st);
 siting_status siting_job_derived(siting_job *job, siting_derived *d);
@@ -318,6 +340,26 @@ siting_status siting_dem_generate(int nrows, double rough_h, int hmin,
                                   int hmax, int nhills, double hill_peak,
                                   double hill_sigma, uint64_t seed,
                                   uint16_t *out);
+
+/* ---- tuning cache --------------------------------------------------------
+ * Kernel shape choices measured by the project's tuner (python -m
+ * siting_amd.tune) for classes of jobs, kept in a text file: one line per
+ * (device, roi class, targets class, edge class), written by the tuner.
+ * The file is the one SITING_TUNE_CACHE names ("off" disables the cache),
+ * else siting_amd/tune/tuning.txt beside the library.  Classes: roi 25
+ * (roi <= 37), 50 (<= 75), 100 (<= 125), 150 (<= 175), 200 (<= 300);
+ * targets 32 (<= 48) or 64; edge 1024 (nrows <= 1536), 2048 (<= 3072) or
+ * 4096. */
+
+/* "<arch>:<CUs>" of a device, e.g. "gfx950:256", the cache's device key. */
+siting_status siting_device_key(int device, char *buf, size_t size);
+/* The cached choices for parameters p on the device with this key:
+ * SITING_OK, or SITING_ENOTFOUND (no cache, or no entry; *out = zeros, the
+ * built-in rules). */
+siting_status siting_tuning_lookup(const char *device_key, const siting_params *p,
+                                   siting_tuning *out);
+/* The cache file in use ("" when disabled). */
+const char *siting_tuning_cache_path(void);
 
 /* ---- library info ----------------------------------------------------- */
 

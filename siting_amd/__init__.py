@@ -72,6 +72,22 @@ class Stats(ctypes.Structure):
         ("ms_findmax", ctypes.c_float),
         ("ms_viewshed", ctypes.c_float),
         ("ms_site", ctypes.c_float),
+        ("rounds", ctypes.c_int),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Tuning(ctypes.Structure):
+    """Kernel shape choices of one job (siting.h, siting_tuning); 0 keeps a
+    field's built-in rule."""
+    _fields_ = [
+        ("vix_halo", ctypes.c_int),
+        ("viewshed_threads", ctypes.c_int),
+        ("viewshed_mode", ctypes.c_int),
+        ("site_q", ctypes.c_int),
+        ("site_tau_shift", ctypes.c_int),
     ]
 
     def as_dict(self) -> dict:
@@ -111,6 +127,11 @@ _SIGS = {
     "siting_dem_generate": (_I, [_I, ctypes.c_double, _I, _I, _I, ctypes.c_double,
                                  ctypes.c_double, ctypes.c_uint64, _VP]),
     "siting_device_count": (_I, []),
+    "siting_job_set_tuning": (_I, [_VP, _VP]),
+    "siting_job_get_tuning": (_I, [_VP, _VP]),
+    "siting_device_key": (_I, [_I, ctypes.c_char_p, ctypes.c_size_t]),
+    "siting_tuning_lookup": (_I, [ctypes.c_char_p, ctypes.POINTER(Params), _VP]),
+    "siting_tuning_cache_path": (ctypes.c_char_p, []),
     "siting_site_it_noalloc": (_I, [_I, _I, _I, _I, _VP, _VP, _VP, _VP,
                                     ctypes.POINTER(Stats), _VP]),
     "siting_batch_create": (_I, [_I, _I, _VP, _VP, ctypes.POINTER(_VP)]),
@@ -170,6 +191,28 @@ def make_params(nrows, roi, height=None, ntests=None, bw=None, nwanted0=None,
     oht = height if oht is None else oht
     tht = height if tht is None else tht
     return Params(nrows, roi, oht, tht, ntests, bw, nwanted0, intervis)
+
+
+def device_key(device: int = 0) -> str:
+    """"<arch>:<CUs>" of a device, the tuning cache's key."""
+    buf = ctypes.create_string_buffer(64)
+    _check(lib().siting_device_key(device, buf, len(buf)))
+    return buf.value.decode()
+
+
+def tuning_lookup(key: str, params: Params):
+    """The tuning cache's choices for these parameters on a device key, or
+    None when the cache holds none."""
+    t = Tuning()
+    rc = lib().siting_tuning_lookup(key.encode(), ctypes.byref(params), ctypes.byref(t))
+    if rc == ENOTFOUND:
+        return None
+    _check(rc)
+    return t
+
+
+def tuning_cache_path() -> str:
+    return (lib().siting_tuning_cache_path() or b"").decode()
 
 
 def derive(p: Params) -> Derived:
@@ -310,6 +353,14 @@ class Job:
         out = np.empty(count, dtype)
         _check(lib().siting_job_download(self._h, which, _ptr(out), out.nbytes))
         return out
+
+    def set_tuning(self, t: "Tuning | None"):
+        _check(lib().siting_job_set_tuning(self._h, ctypes.byref(t) if t is not None else None))
+
+    def get_tuning(self) -> Tuning:
+        t = Tuning()
+        _check(lib().siting_job_get_tuning(self._h, ctypes.byref(t)))
+        return t
 
     def download_into(self, which, out: np.ndarray):
         """Copies the first out.nbytes bytes of buffer `which` into out."""

@@ -5,7 +5,13 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <dlfcn.h>
+
+#include <fstream>
+#include <map>
 #include <mutex>
+#include <sstream>
+#include <string>
 #include <vector>
 
 #include "job.hpp"
@@ -72,6 +78,101 @@ siting_status derive(const siting_params *p, siting_derived *d) {
   if (nwanted > 0x7fffffffLL || nwpb < 1 || lastsize < 1) return SITING_EINVAL;
   // every block must hold at least one cell (blocksize >= 1 guarantees it)
   if (!(bs >= 1.0f)) return SITING_EINVAL;
+  return SITING_OK;
+}
+
+// ---- tuning cache ---------------------------------------------------------
+
+namespace {
+
+struct TuneEntry {
+  std::string key;
+  int roi, ntests, edge;
+  siting_tuning t;
+};
+
+std::mutex g_tune_mu;
+bool g_tune_loaded = false;
+std::string g_tune_path;
+std::vector<TuneEntry> g_tune;
+
+// SITING_TUNE_CACHE, or siting_amd/tune/tuning.txt beside the library
+std::string tune_cache_path() {
+  const char *e = std::getenv("SITING_TUNE_CACHE");
+  if (e && *e) return std::string(e) == "off" ? std::string() : std::string(e);
+  Dl_info info;
+  if (dladdr((void *)&tune_cache_path, &info) && info.dli_fname) {
+    std::string lib = info.dli_fname;
+    const size_t slash = lib.rfind('/');
+    const std::string dir = slash == std::string::npos ? "." : lib.substr(0, slash);
+    return dir + "/../tune/tuning.txt";
+  }
+  return std::string();
+}
+
+// (Re)reads the cache when its path changed; caller holds g_tune_mu.
+void tune_load() {
+  const std::string path = tune_cache_path();
+  if (g_tune_loaded && path == g_tune_path) return;
+  g_tune_loaded = true;
+  g_tune_path = path;
+  g_tune.clear();
+  if (path.empty()) return;
+  std::ifstream f(path);
+  std::string line;
+  while (std::getline(f, line)) {
+    const size_t hash = line.find('#');
+    if (hash != std::string::npos) line.resize(hash);
+    std::istringstream in(line);
+    TuneEntry e{};
+    if (in >> e.key >> e.roi >> e.ntests >> e.edge >> e.t.vix_halo >> e.t.viewshed_threads >>
+        e.t.viewshed_mode >> e.t.site_q >> e.t.site_tau_shift)
+      g_tune.push_back(e);
+  }
+}
+
+int roi_class(int roi) {
+  return roi <= 37 ? 25 : roi <= 75 ? 50 : roi <= 125 ? 100 : roi <= 175 ? 150
+                                                                         : roi <= 300 ? 200 : 0;
+}
+int ntests_class(int ntests) { return ntests <= 48 ? 32 : 64; }
+int edge_class(int n) { return n <= 1536 ? 1024 : n <= 3072 ? 2048 : 4096; }
+
+std::mutex g_key_mu;
+std::map<int, std::string> g_device_keys;
+
+}  // namespace
+
+siting_status tuning_lookup(const char *device_key, const siting_params &p,
+                            siting_tuning *out) {
+  *out = siting_tuning{};
+  if (!device_key) return SITING_EINVAL;
+  const int rc = roi_class(p.roi), tc = ntests_class(p.ntests), ec = edge_class(p.nrows);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  tune_load();
+  for (const TuneEntry &e : g_tune)
+    if (e.key == device_key && e.roi == rc && e.ntests == tc && e.edge == ec) {
+      *out = e.t;
+      return SITING_OK;
+    }
+  return SITING_ENOTFOUND;
+}
+
+// "<arch>:<CUs>" of a device (cached per device).
+siting_status device_key(int device, std::string *out) {
+  std::lock_guard<std::mutex> lk(g_key_mu);
+  auto it = g_device_keys.find(device);
+  if (it != g_device_keys.end()) {
+    *out = it->second;
+    return SITING_OK;
+  }
+  hipDeviceProp_t prop;
+  SITING_HIP(hipGetDeviceProperties(&prop, device));
+  std::string arch = prop.gcnArchName;
+  const size_t colon = arch.find(':');
+  if (colon != std::string::npos) arch.resize(colon);
+  *out = arch + ":" + std::to_string(prop.multiProcessorCount);
+  g_device_keys[device] = *out;
   return SITING_OK;
 }
 
@@ -201,6 +302,10 @@ siting_status job_create_in(int device, const siting_params *p, void *mem,
   j->device = device;
   j->g.p = *p;
   j->g.d = d;
+  {  // the tuning cache's choices for this job, if any
+    std::string key;
+    if (device_key(device, &key) == SITING_OK) tuning_lookup(key.c_str(), *p, &j->g.t);
+  }
   Layout L = layout_for(*p, d);
   j->bytes = L.total;
   auto fail = [&](siting_status s) {
@@ -394,6 +499,7 @@ siting_status siting_job_stats(siting_job *j, siting_stats *st) {
     st->niter = s.niter;
     st->cumarea = (int64_t)s.cumarea;
     st->sum_u = (int64_t)s.sum_u;
+    st->rounds = s.rounds;
     const double n2 = (double)j->g.p.nrows * (double)j->g.p.nrows;
     st->coverage = 100.0 * (double)s.cumarea / n2;
     if (s.niter < 0) return site_loop_failed(s.niter);
@@ -405,6 +511,42 @@ siting_status siting_job_derived(siting_job *j, siting_derived *d) {
   if (!j || !d) return SITING_EINVAL;
   *d = j->g.d;
   return SITING_OK;
+}
+
+siting_status siting_job_set_tuning(siting_job *j, const siting_tuning *t) {
+  if (!j) return SITING_EINVAL;
+  j->g.t = t ? *t : siting_tuning{};
+  return SITING_OK;
+}
+
+siting_status siting_job_get_tuning(siting_job *j, siting_tuning *t) {
+  if (!j || !t) return SITING_EINVAL;
+  *t = j->g.t;
+  return SITING_OK;
+}
+
+siting_status siting_device_key(int device, char *buf, size_t size) {
+  if (!buf || size == 0) return SITING_EINVAL;
+  std::string key;
+  siting_status st = device_key(device, &key);
+  if (st != SITING_OK) return st;
+  if (key.size() + 1 > size) return SITING_EINVAL;
+  std::memcpy(buf, key.c_str(), key.size() + 1);
+  return SITING_OK;
+}
+
+siting_status siting_tuning_lookup(const char *device_key, const siting_params *p,
+                                   siting_tuning *out) {
+  if (!device_key || !p || !out) return SITING_EINVAL;
+  return tuning_lookup(device_key, *p, out);
+}
+
+const char *siting_tuning_cache_path(void) {
+  static thread_local std::string path;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  tune_load();
+  path = g_tune_path;
+  return path.c_str();
 }
 
 // ---------------------------------------------------------------------------
@@ -602,6 +744,7 @@ siting_status siting_site_it(int nrows, int roi, int intervis, int nsheds,
     stats->niter = st.niter;
     stats->cumarea = (int64_t)st.cumarea;
     stats->sum_u = (int64_t)st.sum_u;
+    stats->rounds = st.rounds;
     stats->coverage = 100.0 * (double)st.cumarea / (double)n2;
   }
   if (st.niter < 0) return site_loop_failed(st.niter);
@@ -613,6 +756,34 @@ siting_status siting_site_it(int nrows, int roi, int intervis, int nsheds,
 // ---------------------------------------------------------------------------
 // batches
 // ---------------------------------------------------------------------------
+namespace siting {
+
+// Host form of the work-queue plan (layout in common.hpp).
+struct QueuePlan {
+  std::vector<int> start, size, cls, off, order;
+  int total() const {
+    int t = 0;
+    for (int v : size) t += v;
+    return t;
+  }
+};
+
+// A plan as classes of equal slots, each class with one shared queue.
+struct SlotClasses {
+  std::vector<int> size, count;
+  std::vector<std::vector<int>> queue;
+};
+
+// A plan the batch has run, with each job's loop end and the SITE launch
+// time it measured.
+struct TriedPlan {
+  SlotClasses classes;
+  std::vector<double> end;
+  double span_ms;
+};
+
+}  // namespace siting
+
 struct siting_batch {
   int device = 0, cus = 0;
   std::vector<siting_job *> jobs;
@@ -627,28 +798,20 @@ struct siting_batch {
   unsigned int *ctl = nullptr;         // work-queue state
   unsigned long long *jobt = nullptr;  // [2 * njobs] loop start/end per job
   size_t lds = 0;
-  int slot = 16, nslots = 1, nclasses = 1, total_groups = 16;
-  bool use_graph = false, replan = true, replanned = false;
-  int runs = 0;
+  int slot = 16, nslots = 1, nclasses = 1, total_groups = 16, max_slots = 1;
+  int budget = 16;  // workgroups of the launch (every plan stays within it)
+  bool use_graph = false, replan = true;
+  int runs = 0, runs_seen = 0;
+  // slot tuning: plans tried so far and the current one
+  int tune_left = 0;
+  siting::QueuePlan cur;
+  siting::SlotClasses cur_classes;
+  std::vector<siting::TriedPlan> tried;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
 
 namespace {
-
-// Host form of the work-queue plan (layout in common.hpp).
-// Slot classes of a work-queue plan: a slot for the longest job, big slots
-// and base slots.
-constexpr int kMaxClasses = 3;
-
-struct QueuePlan {
-  std::vector<int> start, size, cls, off, order;
-  int total() const {
-    int t = 0;
-    for (int v : size) t += v;
-    return t;
-  }
-};
 
 // Uploads a plan and re-plans every job for its class's slot size.
 siting_status batch_set_plan(siting_batch *b, const QueuePlan &p,
@@ -674,24 +837,12 @@ siting_status batch_set_plan(siting_batch *b, const QueuePlan &p,
   SITING_HIP(hipMemcpy(b->plan_dev, packed.data(), sizeof(int) * packed.size(),
                        hipMemcpyHostToDevice));
   b->cap = cap;
+  b->cur = p;
   b->lds = lds_max;
   b->nslots = (int)p.start.size();
   b->nclasses = (int)p.off.size() - 1;
   b->total_groups = p.total();
   return SITING_OK;
-}
-
-// nslots slots of `slot` workgroups, one queue in prep order.
-QueuePlan uniform_plan(const siting_batch *b) {
-  QueuePlan p;
-  for (int s = 0; s < b->nslots; s++) {
-    p.start.push_back(s * b->slot);
-    p.size.push_back(b->slot);
-    p.cls.push_back(0);
-  }
-  p.off = {0, (int)b->order.size()};
-  p.order = b->order;
-  return p;
 }
 
 // Every job's VIX, FINDMAX, VIEWSHED and SITE preparation, spread over the
@@ -761,25 +912,57 @@ void batch_drop_graph(siting_batch *b) {
   b->graph = nullptr;
 }
 
-// Slot classes from the measured loop times of the last run.  A job that
-// took t ms on G workgroups is modelled to take t*sqrt(G/G') on G' (the
-// loop's rounds are latency-bound, so more groups help less than linearly).
-// Jobs, longest first, go to whichever slot would finish them first; the
-// mix of an optional slot of 64-128 groups for the longest job, big slots
-// (32 to 128 groups, up to four of them) and slots of
-// the base size with the smallest finish time is kept if it beats the
-// uniform slots by 10 %.  Each class then queues its jobs longest first.
-siting_status batch_replan(siting_batch *b) {
-  const int njobs = (int)b->jobs.size();
-  std::vector<unsigned long long> t((size_t)2 * njobs);
-  SITING_HIP(hipMemcpy(t.data(), b->jobt, sizeof(unsigned long long) * t.size(),
-                       hipMemcpyDeviceToHost));
-  std::vector<double> ms((size_t)njobs), g0((size_t)njobs);
-  for (int i = 0; i < njobs; i++) {
-    const unsigned long long a = t[(size_t)2 * i], e = t[(size_t)2 * i + 1];
-    ms[(size_t)i] = e > a ? (double)(e - a) * 1e-5 : 0.0;  // 100 MHz clock
-    g0[(size_t)i] = site_loop_groups(b->jobs[(size_t)i]->g, b->cus, b->cap[(size_t)i]);
+bool same_plan(const QueuePlan &x, const QueuePlan &y) {
+  return x.start == y.start && x.size == y.size && x.cls == y.cls && x.off == y.off &&
+         x.order == y.order;
+}
+
+// Slot tuning.  The work-queue launch's slots (how many workgroups each
+// job's loop gets, and which jobs take turns on a slot) are tuned on the
+// batch's first runs by measurement, since a loop's time on G workgroups
+// depends on its job (rounds against per-round work) and on what else runs
+// beside it:
+//  - run 1 uses equal slots of the base size;
+//  - run 2 the mixed-slot plan of the square-root model below;
+//  - each later run, up to kTuneRuns, moves the job that finished last in
+//    the fastest plan measured so far onto a slot of its own twice as large
+//    (at least 32 groups), paid for with base slots; a move that does not
+//    shorten the launch is undone, and the next tries the job that finished
+//    last but one, and so on;
+//  - then the fastest plan measured is kept for good.
+// (Alone, a 3365^2 roi-150 job of the uneven batch takes 84 ms on 8 groups,
+// 26 on 32 and 18 on 64; a 2593^2 roi-150 one with 2761 rounds 230, 90 and
+// 77: loops differ widely in how far more groups help.)
+constexpr int kTuneRuns = 5;
+
+void to_plan(const SlotClasses &c, int njobs, int base, QueuePlan *p, std::vector<int> *cap) {
+  *p = QueuePlan{};
+  cap->assign((size_t)njobs, base);
+  p->off = {0};
+  int start = 0;
+  for (size_t k = 0; k < c.size.size(); k++) {
+    for (int n = 0; n < c.count[k]; n++) {
+      p->start.push_back(start);
+      p->size.push_back(c.size[k]);
+      p->cls.push_back((int)k);
+      start += c.size[k];
+    }
+    for (int i : c.queue[k]) {
+      p->order.push_back(i);
+      (*cap)[(size_t)i] = c.size[k];
+    }
+    p->off.push_back((int)p->order.size());
   }
+}
+
+// The mixed-slot plan from the first (uniform) run: a job that took t ms on
+// G0 groups is modelled to take t sqrt(G0 / G) on G; the mix of an optional
+// slot of 64-128 groups for the longest job, up to four big slots (32 to
+// 128 groups) and base slots with the smallest modelled span is kept if it
+// beats equal slots by 10 %; one queue per slot size, longest jobs first.
+bool sqrt_plan(siting_batch *b, const std::vector<double> &ms,
+               const std::vector<double> &g0, SlotClasses *out) {
+  const int njobs = (int)b->jobs.size();
   auto time_on = [&](int i, int groups) {
     const double g = site_loop_groups(b->jobs[(size_t)i]->g, b->cus, groups);
     return ms[(size_t)i] * std::sqrt(g0[(size_t)i] / g);
@@ -788,9 +971,7 @@ siting_status batch_replan(siting_batch *b) {
   for (int i = 0; i < njobs; i++) by_time[(size_t)i] = i;
   std::stable_sort(by_time.begin(), by_time.end(),
                    [&](int x, int y) { return ms[(size_t)x] > ms[(size_t)y]; });
-  const int total = b->total_groups;
-  // slot sizes of a candidate plan: an optional slot of s1 groups for the
-  // longest job, nb slots of sb groups, and base slots for the rest
+  const int total = b->budget;
   auto sizes_of = [&](int s1, int nb, int sb) {
     std::vector<int> sz;
     if (s1 > 0) sz.push_back(s1);
@@ -799,8 +980,6 @@ siting_status batch_replan(siting_batch *b) {
     for (int k = 0; k < (total - used) / b->slot; k++) sz.push_back(b->slot);
     return sz;
   };
-  // list schedule over slots of the given sizes; returns the makespan and
-  // fills each job's slot size
   auto schedule = [&](const std::vector<int> &size, std::vector<int> *slot_of) {
     std::vector<double> fin(size.size(), 0.0);
     if (slot_of) slot_of->assign((size_t)njobs, b->slot);
@@ -836,36 +1015,129 @@ siting_status batch_replan(siting_batch *b) {
           best_sizes = sz;
         }
       }
-  if (best_sizes.empty() || best_m > 0.9 * m0) return SITING_OK;
+  if (best_sizes.empty() || best_m > 0.9 * m0) return false;
   std::vector<int> slot_of;
   schedule(best_sizes, &slot_of);
-  // one class (queue) per distinct slot size, in the slots' order
-  std::vector<int> class_size;
-  QueuePlan p;
-  int start = 0;
+  SlotClasses c;
   for (int sz : best_sizes) {
-    int c = 0;
-    while (c < (int)class_size.size() && class_size[(size_t)c] != sz) c++;
-    if (c == (int)class_size.size()) class_size.push_back(sz);
-    p.start.push_back(start);
-    p.size.push_back(sz);
-    p.cls.push_back(c);
-    start += sz;
+    size_t k = 0;
+    while (k < c.size.size() && c.size[k] != sz) k++;
+    if (k == c.size.size()) {
+      c.size.push_back(sz);
+      c.count.push_back(0);
+      c.queue.push_back({});
+    }
+    c.count[k]++;
   }
-  std::vector<std::vector<int>> queues(class_size.size());
-  std::vector<int> cap((size_t)njobs);
   for (int i : by_time) {
-    int c = 0;
-    while (class_size[(size_t)c] != slot_of[(size_t)i]) c++;
-    queues[(size_t)c].push_back(i);
-    cap[(size_t)i] = slot_of[(size_t)i];
+    size_t k = 0;
+    while (c.size[k] != slot_of[(size_t)i]) k++;
+    c.queue[k].push_back(i);
   }
-  p.off = {0};
-  for (const auto &q : queues) {
-    p.order.insert(p.order.end(), q.begin(), q.end());
-    p.off.push_back((int)p.order.size());
+  *out = c;
+  return true;
+}
+
+// Moves job j onto a slot of its own twice its current size (at least 32
+// groups, at most CUs/2), paid for with base slots; false if it cannot.
+bool move_to_own_slot(const siting_batch *b, SlotClasses *c, int j) {
+  size_t from = 0;
+  for (; from < c->queue.size(); from++) {
+    auto &q = c->queue[from];
+    if (std::find(q.begin(), q.end(), j) != q.end()) break;
   }
+  if (from == c->queue.size()) return false;
+  const int cur = c->size[from];
+  const int want = std::min(std::max(2 * cur, 32), b->cus / 2);
+  if (want <= cur) return false;
+  // base class: the smallest slots
+  size_t base = 0;
+  for (size_t k = 1; k < c->size.size(); k++)
+    if (c->size[k] < c->size[base]) base = k;
+  // a job alone in its class gives its slot back
+  const bool alone = c->queue[from].size() == 1 && c->count[from] == 1;
+  int need = want - (alone ? cur : 0);
+  if (need % c->size[base]) return false;
+  const int take = need / c->size[base];
+  if (base == from && alone) return false;
+  if (c->count[base] - take < 1 || (base == from && c->count[base] - take < 1)) return false;
+  c->count[base] -= take;
+  if (alone) {
+    c->size[from] = want;
+    return true;
+  }
+  auto &q = c->queue[from];
+  q.erase(std::find(q.begin(), q.end(), j));
+  c->size.push_back(want);
+  c->count.push_back(1);
+  c->queue.push_back({j});
+  return true;
+}
+
+// After a synced run: record the plan's measured launch time and job end
+// times, and choose the next plan.
+siting_status batch_tune(siting_batch *b) {
+  const int njobs = (int)b->jobs.size();
+  std::vector<unsigned long long> t((size_t)2 * njobs);
+  SITING_HIP(hipMemcpy(t.data(), b->jobt, sizeof(unsigned long long) * t.size(),
+                       hipMemcpyDeviceToHost));
+  unsigned long long first = ~0ull;
+  for (int i = 0; i < njobs; i++) first = std::min(first, t[(size_t)2 * i]);
+  std::vector<double> end((size_t)njobs), ms((size_t)njobs), g((size_t)njobs);
+  for (int i = 0; i < njobs; i++) {
+    const unsigned long long a = t[(size_t)2 * i], e = t[(size_t)2 * i + 1];
+    end[(size_t)i] = e > first ? (double)(e - first) * 1e-5 : 0.0;  // 100 MHz clock
+    ms[(size_t)i] = e > a ? (double)(e - a) * 1e-5 : 0.0;
+    g[(size_t)i] = site_loop_groups(b->jobs[(size_t)i]->g, b->cus, b->cap[(size_t)i]);
+  }
+  float span = 0;
+  SITING_HIP(hipEventElapsedTime(&span, b->ts, b->t1));
+  b->tried.push_back({b->cur_classes, end, (double)span});
+  b->tune_left--;
+  size_t best = 0;
+  for (size_t k = 1; k < b->tried.size(); k++)
+    if (b->tried[k].span_ms < b->tried[best].span_ms) best = k;
+  SlotClasses next;
+  bool have = false;
+  if (b->tune_left > 0) {
+    if (b->tried.size() == 1) {
+      have = sqrt_plan(b, ms, g, &next);
+    }
+    // local moves from the best plan: its jobs by end time, latest first,
+    // skipping the moves already tried from it
+    const auto &bp = b->tried[best];
+    std::vector<int> late((size_t)njobs);
+    for (int i = 0; i < njobs; i++) late[(size_t)i] = i;
+    std::stable_sort(late.begin(), late.end(),
+                     [&](int x, int y) { return bp.end[(size_t)x] > bp.end[(size_t)y]; });
+    for (size_t r = 0; !have && r < late.size() && r < 8; r++) {
+      SlotClasses c = bp.classes;
+      if (!move_to_own_slot(b, &c, late[r])) continue;
+      bool seen = false;
+      for (const auto &tp : b->tried)
+        if (tp.classes.size == c.size && tp.classes.count == c.count &&
+            tp.classes.queue == c.queue)
+          seen = true;
+      if (seen) continue;
+      next = c;
+      have = true;
+    }
+  }
+  if (!have) {
+    b->tune_left = 0;
+    next = b->tried[best].classes;
+  }
+  QueuePlan p;
+  std::vector<int> cap;
+  to_plan(next, njobs, b->slot, &p, &cap);
+  if ((int)p.start.size() > b->max_slots) {
+    b->tune_left = 0;
+    next = b->tried[best].classes;
+    to_plan(next, njobs, b->slot, &p, &cap);
+  }
+  if (same_plan(p, b->cur)) return SITING_OK;
   batch_drop_graph(b);
+  b->cur_classes = next;
   return batch_set_plan(b, p, cap);
 }
 
@@ -910,6 +1182,12 @@ siting_status siting_batch_create(int device, int njobs, const siting_params *pa
     siting_status st = siting_job_create(device, &params[i], &j);
     if (st != SITING_OK) return fail(st);
     j->site_state = true;  // stats read the SITE state
+    // the tuning cache's SITE choices are measured for a job's loop alone on
+    // up to half the CUs; in work-queue slots of 4-128 groups beside 60
+    // other loops the built-in rules measured faster (uneven 64-job batch:
+    // SITE launch 96 ms with the rules, 116 with the cached choices)
+    j->g.t.site_q = 0;
+    j->g.t.site_tau_shift = 0;
     b->jobs.push_back(j);
   }
   SITING_HIP(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device));
@@ -926,6 +1204,7 @@ siting_status siting_batch_create(int device, int njobs, const siting_params *pa
   if (b->nslots > njobs) b->nslots = njobs;
   b->use_graph = opts && opts->use_graph;
   b->replan = !(opts && opts->fixed_slots);
+  b->tune_left = kTuneRuns;
   hipError_t e;
   for (int k = 0; k < nstreams; k++) {
     hipStream_t s = nullptr;
@@ -951,17 +1230,24 @@ siting_status siting_batch_create(int device, int njobs, const siting_params *pa
                       (double)b->jobs[(size_t)i]->g.d.shed_words;
   std::stable_sort(b->order.begin(), b->order.end(),
                    [&](int x, int y) { return work[(size_t)x] > work[(size_t)y]; });
-  // buffers sized for any plan the re-planning may choose: at most
-  // nslots slots (big slots only replace base ones) and two classes
-  const int max_slots = b->nslots;
+  // buffers sized for any plan the re-planning may choose: slots of at
+  // least 4 workgroups, each with its own queue
+  const int max_slots = std::max(b->nslots, b->nslots * b->slot / 4);
+  b->max_slots = max_slots;
   if ((e = hipMalloc(&b->args_dev, sizeof(SiteArgs) * (size_t)njobs)) != hipSuccess ||
       (e = hipMalloc(&b->plan_dev,
-                     sizeof(int) * site_queue_plan_ints(max_slots, kMaxClasses, njobs))) !=
+                     sizeof(int) * site_queue_plan_ints(max_slots, max_slots, njobs))) !=
           hipSuccess ||
-      (e = hipMalloc(&b->ctl, 4 * site_queue_ctl_words(max_slots, kMaxClasses))) != hipSuccess ||
+      (e = hipMalloc(&b->ctl, 4 * site_queue_ctl_words(max_slots, max_slots))) != hipSuccess ||
       (e = hipMalloc(&b->jobt, sizeof(unsigned long long) * 2 * (size_t)njobs)) != hipSuccess)
     return fail(hip_fail(e, "hipMalloc(batch)", __FILE__, __LINE__));
-  siting_status st = batch_set_plan(b, uniform_plan(b), std::vector<int>((size_t)njobs, b->slot));
+  b->budget = b->nslots * b->slot;
+  // first plan: equal slots of the base size, one queue in prep order
+  b->cur_classes = SlotClasses{{b->slot}, {b->nslots}, {b->order}};
+  QueuePlan p0;
+  std::vector<int> cap0;
+  to_plan(b->cur_classes, njobs, b->slot, &p0, &cap0);
+  siting_status st = batch_set_plan(b, p0, cap0);
   if (st != SITING_OK) return fail(st);
   *out = b;
   return SITING_OK;
@@ -995,11 +1281,13 @@ siting_status siting_batch_sync(siting_batch *b) {
   if (!b) return SITING_EINVAL;
   SITING_HIP(hipSetDevice(b->device));
   for (hipStream_t s : b->streams) SITING_HIP(hipStreamSynchronize(s));
-  if (b->replan && !b->replanned && b->runs > 0) {
-    b->replanned = true;
-    siting_status st = batch_replan(b);
-    if (st != SITING_OK) return st;
-    if (b->use_graph && !b->exec) return batch_capture(b);
+  if (b->runs > b->runs_seen) {
+    b->runs_seen = b->runs;
+    if (b->replan && b->tune_left > 0) {
+      siting_status st = batch_tune(b);
+      if (st != SITING_OK) return st;
+      if (b->use_graph && !b->exec) return batch_capture(b);
+    }
   }
   return SITING_OK;
 }

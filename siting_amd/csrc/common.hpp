@@ -96,7 +96,7 @@ struct SiteState {
   int nused;
   int niter;
   unsigned int barrier;  // grid barrier arrival counter
-  int pad;
+  int rounds;            // rounds of the persistent loop (picks are batched)
 };
 
 // ---------------------------------------------------------------------------
@@ -105,7 +105,12 @@ struct SiteState {
 struct Geometry {
   siting_params p;
   siting_derived d;
+  siting_tuning t;  // kernel shape choices (zeros: the built-in rules)
 };
+
+// Class of a job in the tuning cache, and the lookup (api.hip).
+siting_status tuning_lookup(const char *device_key, const siting_params &p,
+                            siting_tuning *out);
 
 siting_status derive(const siting_params *p, siting_derived *d);
 

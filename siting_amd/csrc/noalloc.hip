@@ -137,6 +137,7 @@ siting_status siting_site_it_noalloc(int nrows, int roi, int intervis, int nshed
   stats->niter = h.niter;
   stats->cumarea = (int64_t)h.cumarea;
   stats->sum_u = (int64_t)h.sum_u;
+  stats->rounds = h.rounds;
   stats->coverage = 100.0 * (double)h.cumarea / ((double)nrows * (double)nrows);
   if (h.niter < 0) return site_loop_failed(h.niter);
   return SITING_OK;

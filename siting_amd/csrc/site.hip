@@ -54,6 +54,7 @@
 // Barriers are plain atomics on a counter, so every workgroup of a loop must
 // be resident at once: the host caps G at half the CUs (one workgroup per
 // CU), and the work-queue launch sizes its slots the same way.
+#include <algorithm>
 #include <cstdlib>
 
 #include "common.hpp"
@@ -197,6 +198,7 @@ __global__ __launch_bounds__(256) void site_first_pass(SiteArgs a) {
     a.st->sum_u = (unsigned long long)a.nsheds;
     a.st->nused = 0;
     a.st->niter = 1;
+    a.st->rounds = 0;
     a.st->barrier = 0;
   }
   if (c >= a.nsheds) return;
@@ -1089,6 +1091,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   if (g == 0 && tid == 0) {
     a.st->nused = nused;
     a.st->niter = niter;
+    a.st->rounds = ph_round;
     a.st->cumarea = (unsigned long long)cumarea;
   }
 }
@@ -1156,13 +1159,15 @@ struct Plan {
 // Column classes per cell row.  More classes spread a pick's update window
 // over more workgroups but lengthen the selection scan; three pay only when
 // a pick's expected update list pi*(2roi)^2 * nsheds/n^2 reaches 3000
-// (measured).  SITING_SITE_Q overrides.
-int choose_q(int n, int roi, int nsheds) {
+// (measured).  A job's tuning (2 or 3) replaces the rule; SITING_SITE_Q
+// overrides both.
+int choose_q(int n, int roi, int nsheds, int tuned = 0) {
   const char *e = std::getenv("SITING_SITE_Q");
   if (e && *e) {
     const int q = std::atoi(e);
     return q < 1 ? 1 : (q > 8 ? 8 : q);
   }
+  if (tuned == 2 || tuned == 3) return tuned;
   const double list = 3.14159265358979 * 4.0 * (double)roi * roi * (double)nsheds /
                       ((double)n * (double)n);
   return list >= 3000.0 ? 3 : 2;
@@ -1170,10 +1175,11 @@ int choose_q(int n, int roi, int nsheds) {
 
 // Update candidates whose area is below tau = head - head / 2^s (head: the
 // round's largest key's area) are not decremented but kept as stale bounds
-// (SITING_SITE_TAU_SHIFT sets s; 0 turns it off).
-int tau_shift() {
+// (s = 5; a job's tuning may choose 1..20; SITING_SITE_TAU_SHIFT overrides,
+// 0 turning it off).
+int tau_shift(int tuned = 0) {
   const char *e = std::getenv("SITING_SITE_TAU_SHIFT");
-  const int t = e && *e ? std::atoi(e) : 5;
+  const int t = e && *e ? std::atoi(e) : (tuned >= 1 && tuned <= 20 ? tuned : 5);
   return t < 0 ? 0 : (t > 20 ? 20 : t);
 }
 
@@ -1204,11 +1210,12 @@ long long stop_area(int n) {
   return lo;
 }
 
-Plan make_plan(int n, int roi, int nsheds, int groups_cap, int fixed_groups = 0) {
+Plan make_plan(int n, int roi, int nsheds, int groups_cap, int fixed_groups = 0,
+               int q_tuned = 0) {
   Plan p{};
   p.cs = cell_edge(n, roi);
   p.ncr = (n + p.cs - 1) / p.cs;
-  p.Q = choose_q(n, roi, nsheds);
+  p.Q = choose_q(n, roi, nsheds, q_tuned);
   if (p.Q > p.ncr) p.Q = p.ncr;
   p.nunits = p.ncr * p.Q;
   // units in a pick's 4*roi window, times workgroups per such unit, and at
@@ -1303,11 +1310,16 @@ size_t scratch_layout(const Plan &p, int nsheds, uint8_t *base, Scratch *out) {
 
 }  // namespace
 
-// The plan's group count depends on the device; the scratch size is taken
-// for the largest group count, which bounds every smaller one.
+// The plan's group count depends on the device and its column classes on
+// the job's tuning; the scratch size is taken for the largest group count,
+// which bounds every smaller one, and the largest over the built-in rule and
+// both tunable class counts.
 size_t site_scratch_bytes(int nrows, int roi, int nsheds) {
-  return scratch_layout(make_plan(nrows, roi, nsheds, kMaxGroups), nsheds, nullptr,
-                        nullptr);
+  size_t m = 0;
+  for (int q : {0, 2, 3})
+    m = std::max(m, scratch_layout(make_plan(nrows, roi, nsheds, kMaxGroups, 0, q), nsheds,
+                                   nullptr, nullptr));
+  return m;
 }
 
 int site_key_rows(int nrows, int roi) { return 1; }
@@ -1316,7 +1328,7 @@ siting_status site_make_args(const Geometry &g, const int32_t *obs,
                              const uint64_t *sheds, const SiteBuffers &b, int slot,
                              SiteArgs *out, size_t *lds) {
   const int n = g.p.nrows, roi = g.p.roi, nsheds = g.d.nwanted;
-  const Plan p = make_plan(n, roi, nsheds, slot, slot);
+  const Plan p = make_plan(n, roi, nsheds, slot, slot, g.t.site_q);
   Scratch sc;
   scratch_layout(p, nsheds, b.scratch, &sc);
   SiteArgs a{};
@@ -1367,7 +1379,7 @@ siting_status site_make_args(const Geometry &g, const int32_t *obs,
   a.o_t = p.o_t;
   a.o_used = p.o_used;
   a.o_seen = p.o_seen;
-  a.tau_shift = tau_shift();
+  a.tau_shift = tau_shift(g.t.site_tau_shift);
   a.stop_area = stop_area(n);
   // (a test forces the give-up path with a tiny limit, every kernel still
   // exiting normally)
@@ -1397,7 +1409,7 @@ siting_status site_launch_prep(const SiteArgs &a, hipStream_t s) {
 int site_loop_groups(const Geometry &g, int device_cus, int groups_cap) {
   int cap = device_cus / 2 > 0 ? device_cus / 2 : 1;
   if (groups_cap > 0 && groups_cap < cap) cap = groups_cap;
-  return make_plan(g.p.nrows, g.p.roi, g.d.nwanted, cap).G;
+  return make_plan(g.p.nrows, g.p.roi, g.d.nwanted, cap, 0, g.t.site_q).G;
 }
 
 siting_status launch_site(const Geometry &g, const int32_t *obs,
@@ -1415,7 +1427,7 @@ siting_status launch_site(const Geometry &g, const int32_t *obs,
   if (groups_cap > 0 && groups_cap < cap) cap = groups_cap;
   if (const char *e = std::getenv("SITING_SITE_GROUPS"))  // tuning override
     if (std::atoi(e) > 0 && std::atoi(e) < cap) cap = std::atoi(e);
-  const Plan p = make_plan(g.p.nrows, g.p.roi, g.d.nwanted, cap);
+  const Plan p = make_plan(g.p.nrows, g.p.roi, g.d.nwanted, cap, 0, g.t.site_q);
   SiteArgs a;
   size_t lds = 0;
   siting_status st = site_make_args(g, obs, sheds, b, p.G, &a, &lds);

@@ -500,15 +500,22 @@ siting_status launch_viewshed(const Geometry &g, const uint16_t *elevs, uint16_t
                      elevs_t, g.p.nrows);
   const int per_xcd = (nsheds + 7) / 8;
   const int words = g.d.nr * g.d.nwpr;
-  // 256 threads (4 wavefronts) per observer, rays dealt round-robin; the
-  // count can be overridden for tuning
+  // 256 threads (4 wavefronts) per observer, rays dealt by box side; a
+  // job's tuning replaces the count, SITING_VIEWSHED_THREADS overrides both
   int threads = 256;
+  auto valid_threads = [](int t) { return t >= kWave && t <= kMaxThreads && t % kWave == 0; };
+  if (valid_threads(g.t.viewshed_threads)) threads = g.t.viewshed_threads;
   if (const char *e = std::getenv("SITING_VIEWSHED_THREADS")) {
     const int t = std::atoi(e);
-    if (t >= kWave && t <= kMaxThreads && t % kWave == 0) threads = t;
+    if (valid_threads(t)) threads = t;
   }
   const size_t split_bytes = 2 * sizeof(uint32_t) * g.d.nr * (size_t)split_stride(g.d.nwpr);
-  const int mode = vs_split(g.p.roi) ? kSplit : words <= kLdsWords ? kLds : kGlobal;
+  // bitsets: the split pair when it fits, else one LDS bitset when it fits,
+  // else HBM; a job's tuning may pick another mode that fits
+  int mode = vs_split(g.p.roi) ? kSplit : words <= kLdsWords ? kLds : kGlobal;
+  if (g.t.viewshed_mode == 1 && vs_split(g.p.roi)) mode = kSplit;
+  else if (g.t.viewshed_mode == 2 && words <= kLdsWords) mode = kLds;
+  else if (g.t.viewshed_mode == 3) mode = kGlobal;
   const size_t lds = mode == kSplit ? split_bytes
                      : mode == kLds ? sizeof(uint64_t) * words : 0;
 #define SITING_VS(A)                                                               \

@@ -259,9 +259,10 @@ siting_status launch_vix(const Geometry &g, const uint16_t *elevs, uint8_t *vix,
   const int per_xcd = (ntiles + 7) / 8;
   // LDS halo: 32 cells when samples reach 32 steps and a tile has enough
   // pairs to repay the larger window (measured: roi 100-150 with 64 targets
-  // 2-6 % faster, roi 25 or 32 targets slower), else 16 (tuning override
-  // SITING_VIX_HALO)
+  // 2-6 % faster, roi 25 or 32 targets slower), else 16; a job's tuning
+  // (8, 16 or 32) replaces the rule, SITING_VIX_HALO overrides both
   int halo = g.p.roi >= 64 && g.p.ntests >= 64 ? 32 : 16;
+  if (g.t.vix_halo == 8 || g.t.vix_halo == 16 || g.t.vix_halo == 32) halo = g.t.vix_halo;
   if (const char *e = std::getenv("SITING_VIX_HALO")) halo = std::atoi(e);
   const bool rlds = g.p.roi <= kRecipLdsMax;
   const size_t base = sizeof(uint32_t) * (4 * (size_t)g.p.ntests + kThreads);
