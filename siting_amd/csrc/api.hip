@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "job.hpp"
+#include "slot_plan.hpp"
 
 namespace siting {
 
@@ -758,22 +759,6 @@ siting_status siting_site_it(int nrows, int roi, int intervis, int nsheds,
 // ---------------------------------------------------------------------------
 namespace siting {
 
-// Host form of the work-queue plan (layout in common.hpp).
-struct QueuePlan {
-  std::vector<int> start, size, cls, off, order;
-  int total() const {
-    int t = 0;
-    for (int v : size) t += v;
-    return t;
-  }
-};
-
-// A plan as classes of equal slots, each class with one shared queue.
-struct SlotClasses {
-  std::vector<int> size, count;
-  std::vector<std::vector<int>> queue;
-};
-
 // A plan the batch has run, with each job's loop end and the SITE launch
 // time it measured.
 struct TriedPlan {
@@ -912,11 +897,6 @@ void batch_drop_graph(siting_batch *b) {
   b->graph = nullptr;
 }
 
-bool same_plan(const QueuePlan &x, const QueuePlan &y) {
-  return x.start == y.start && x.size == y.size && x.cls == y.cls && x.off == y.off &&
-         x.order == y.order;
-}
-
 // Slot tuning.  The work-queue launch's slots (how many workgroups each
 // job's loop gets, and which jobs take turns on a slot) are tuned on the
 // batch's first runs by measurement, since a loop's time on G workgroups
@@ -934,145 +914,6 @@ bool same_plan(const QueuePlan &x, const QueuePlan &y) {
 // 26 on 32 and 18 on 64; a 2593^2 roi-150 one with 2761 rounds 230, 90 and
 // 77: loops differ widely in how far more groups help.)
 constexpr int kTuneRuns = 5;
-
-void to_plan(const SlotClasses &c, int njobs, int base, QueuePlan *p, std::vector<int> *cap) {
-  *p = QueuePlan{};
-  cap->assign((size_t)njobs, base);
-  p->off = {0};
-  int start = 0;
-  for (size_t k = 0; k < c.size.size(); k++) {
-    for (int n = 0; n < c.count[k]; n++) {
-      p->start.push_back(start);
-      p->size.push_back(c.size[k]);
-      p->cls.push_back((int)k);
-      start += c.size[k];
-    }
-    for (int i : c.queue[k]) {
-      p->order.push_back(i);
-      (*cap)[(size_t)i] = c.size[k];
-    }
-    p->off.push_back((int)p->order.size());
-  }
-}
-
-// The mixed-slot plan from the first (uniform) run: a job that took t ms on
-// G0 groups is modelled to take t sqrt(G0 / G) on G; the mix of an optional
-// slot of 64-128 groups for the longest job, up to four big slots (32 to
-// 128 groups) and base slots with the smallest modelled span is kept if it
-// beats equal slots by 10 %; one queue per slot size, longest jobs first.
-bool sqrt_plan(siting_batch *b, const std::vector<double> &ms,
-               const std::vector<double> &g0, SlotClasses *out) {
-  const int njobs = (int)b->jobs.size();
-  auto time_on = [&](int i, int groups) {
-    const double g = site_loop_groups(b->jobs[(size_t)i]->g, b->cus, groups);
-    return ms[(size_t)i] * std::sqrt(g0[(size_t)i] / g);
-  };
-  std::vector<int> by_time((size_t)njobs);
-  for (int i = 0; i < njobs; i++) by_time[(size_t)i] = i;
-  std::stable_sort(by_time.begin(), by_time.end(),
-                   [&](int x, int y) { return ms[(size_t)x] > ms[(size_t)y]; });
-  const int total = b->budget;
-  auto sizes_of = [&](int s1, int nb, int sb) {
-    std::vector<int> sz;
-    if (s1 > 0) sz.push_back(s1);
-    for (int k = 0; k < nb; k++) sz.push_back(sb);
-    const int used = (s1 > 0 ? s1 : 0) + nb * sb;
-    for (int k = 0; k < (total - used) / b->slot; k++) sz.push_back(b->slot);
-    return sz;
-  };
-  auto schedule = [&](const std::vector<int> &size, std::vector<int> *slot_of) {
-    std::vector<double> fin(size.size(), 0.0);
-    if (slot_of) slot_of->assign((size_t)njobs, b->slot);
-    for (int i : by_time) {
-      size_t best = 0;
-      double bf = 1e300;
-      for (size_t k = 0; k < size.size(); k++) {
-        const double f = fin[k] + time_on(i, size[k]);
-        if (f < bf) {
-          bf = f;
-          best = k;
-        }
-      }
-      fin[best] = bf;
-      if (slot_of) (*slot_of)[(size_t)i] = size[best];
-    }
-    double m = 0;
-    for (double f : fin) m = f > m ? f : m;
-    return m;
-  };
-  const double m0 = schedule(sizes_of(0, 0, b->slot), nullptr);
-  std::vector<int> best_sizes;
-  double best_m = m0;
-  for (int s1 : {0, 64, 96, 128})
-    for (int sb : {32, 48, 64, 96, 128})
-      for (int nb = 0; nb <= 4; nb++) {
-        if (sb <= b->slot || (s1 > 0 ? s1 : 0) + nb * sb > total - b->slot) continue;
-        if (s1 == 0 && nb == 0) continue;
-        const std::vector<int> sz = sizes_of(s1, nb, sb);
-        const double m = schedule(sz, nullptr);
-        if (m < best_m) {
-          best_m = m;
-          best_sizes = sz;
-        }
-      }
-  if (best_sizes.empty() || best_m > 0.9 * m0) return false;
-  std::vector<int> slot_of;
-  schedule(best_sizes, &slot_of);
-  SlotClasses c;
-  for (int sz : best_sizes) {
-    size_t k = 0;
-    while (k < c.size.size() && c.size[k] != sz) k++;
-    if (k == c.size.size()) {
-      c.size.push_back(sz);
-      c.count.push_back(0);
-      c.queue.push_back({});
-    }
-    c.count[k]++;
-  }
-  for (int i : by_time) {
-    size_t k = 0;
-    while (c.size[k] != slot_of[(size_t)i]) k++;
-    c.queue[k].push_back(i);
-  }
-  *out = c;
-  return true;
-}
-
-// Moves job j onto a slot of its own twice its current size (at least 32
-// groups, at most CUs/2), paid for with base slots; false if it cannot.
-bool move_to_own_slot(const siting_batch *b, SlotClasses *c, int j) {
-  size_t from = 0;
-  for (; from < c->queue.size(); from++) {
-    auto &q = c->queue[from];
-    if (std::find(q.begin(), q.end(), j) != q.end()) break;
-  }
-  if (from == c->queue.size()) return false;
-  const int cur = c->size[from];
-  const int want = std::min(std::max(2 * cur, 32), b->cus / 2);
-  if (want <= cur) return false;
-  // base class: the smallest slots
-  size_t base = 0;
-  for (size_t k = 1; k < c->size.size(); k++)
-    if (c->size[k] < c->size[base]) base = k;
-  // a job alone in its class gives its slot back
-  const bool alone = c->queue[from].size() == 1 && c->count[from] == 1;
-  int need = want - (alone ? cur : 0);
-  if (need % c->size[base]) return false;
-  const int take = need / c->size[base];
-  if (base == from && alone) return false;
-  if (c->count[base] - take < 1 || (base == from && c->count[base] - take < 1)) return false;
-  c->count[base] -= take;
-  if (alone) {
-    c->size[from] = want;
-    return true;
-  }
-  auto &q = c->queue[from];
-  q.erase(std::find(q.begin(), q.end(), j));
-  c->size.push_back(want);
-  c->count.push_back(1);
-  c->queue.push_back({j});
-  return true;
-}
 
 // After a synced run: record the plan's measured launch time and job end
 // times, and choose the next plan.
@@ -1101,7 +942,11 @@ siting_status batch_tune(siting_batch *b) {
   bool have = false;
   if (b->tune_left > 0) {
     if (b->tried.size() == 1) {
-      have = sqrt_plan(b, ms, g, &next);
+      have = sqrt_plan(ms, g,
+                       [&](int i, int size) {
+                         return (double)site_loop_groups(b->jobs[(size_t)i]->g, b->cus, size);
+                       },
+                       b->budget, b->slot, &next);
     }
     // local moves from the best plan: its jobs by end time, latest first,
     // skipping the moves already tried from it
@@ -1112,7 +957,7 @@ siting_status batch_tune(siting_batch *b) {
                      [&](int x, int y) { return bp.end[(size_t)x] > bp.end[(size_t)y]; });
     for (size_t r = 0; !have && r < late.size() && r < 8; r++) {
       SlotClasses c = bp.classes;
-      if (!move_to_own_slot(b, &c, late[r])) continue;
+      if (!move_to_own_slot(b->cus / 2, &c, late[r])) continue;
       bool seen = false;
       for (const auto &tp : b->tried)
         if (tp.classes.size == c.size && tp.classes.count == c.count &&
