@@ -101,6 +101,9 @@ typedef struct {
   int site_q;            /* SITE: column classes per cell row, 2 or 3 */
   int site_tau_shift;    /* SITE: update candidates below head - head/2^s
                             become stale bounds, s in 1..20 */
+  int site_walk_evals;   /* SITE, intervis: 1 + the exact area computations
+                            a round's pick walk may make to admit another
+                            pick (1: none), 1..33 */
 } siting_tuning;
 
 /* ---- parameters ------------------------------------------------------- */

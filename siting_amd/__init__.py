@@ -88,6 +88,7 @@ class Tuning(ctypes.Structure):
         ("viewshed_mode", ctypes.c_int),
         ("site_q", ctypes.c_int),
         ("site_tau_shift", ctypes.c_int),
+        ("site_walk_evals", ctypes.c_int),
     ]
 
     def as_dict(self) -> dict:

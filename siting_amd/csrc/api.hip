@@ -127,8 +127,10 @@ void tune_load() {
     std::istringstream in(line);
     TuneEntry e{};
     if (in >> e.key >> e.roi >> e.ntests >> e.edge >> e.t.vix_halo >> e.t.viewshed_threads >>
-        e.t.viewshed_mode >> e.t.site_q >> e.t.site_tau_shift)
+        e.t.viewshed_mode >> e.t.site_q >> e.t.site_tau_shift) {
+      if (!(in >> e.t.site_walk_evals)) e.t.site_walk_evals = 0;  // older lines
       g_tune.push_back(e);
+    }
   }
 }
 
@@ -218,7 +220,7 @@ Layout layout_for(const siting_params &p, const siting_derived &d) {
   put(L.recip, ((size_t)p.roi + 1) * 4);
   put(L.jump, 4 * (size_t)p.ntests * 4);
   put(L.vstab, vs_tables(p.roi).bytes);
-  put(L.scratch, site_scratch_bytes(p.nrows, p.roi, d.nwanted));
+  put(L.scratch, site_scratch_bytes(p.nrows, p.roi, d.nwanted, p.intervis));
   L.total = off;
   return L;
 }
@@ -728,7 +730,7 @@ siting_status siting_site_it(int nrows, int roi, int intervis, int nsheds,
   CHECK(dalloc(drm, 2 * (size_t)site_key_rows(nrows, roi) * 8));
   CHECK(dalloc(dord, nw * 4));
   CHECK(dalloc(dst, sizeof(SiteState)));
-  CHECK(dalloc(dscr, site_scratch_bytes(nrows, roi, nsheds)));
+  CHECK(dalloc(dscr, site_scratch_bytes(nrows, roi, nsheds, intervis)));
   SITING_HIP(hipMemcpyAsync(dobs.p, obs, nw * 4, hipMemcpyHostToDevice, s));
   SITING_HIP(hipMemcpyAsync(dsh.p, sheds, sbytes, hipMemcpyHostToDevice, s));
   SiteBuffers b{(int32_t *)darea.p, (uint8_t *)dused.p, (uint64_t *)dcum.p,

@@ -211,6 +211,8 @@ struct SiteArgs {
   uint64_t *nb;      // [kSiteMaxBatch][nr][nwpr + 1] bits each pick newly covered
   unsigned int *sseen;  // [G][nsheds/32 + 1] seen bitmaps of global-mode groups
   uint64_t *phash;   // [2][kMaxGroups] per-round hash of each workgroup's picks
+  int32_t *sB;       // intervis: [nsheds] area bounds of global-mode workgroups
+  uint64_t *cbnd;    // intervis: [2][ncr][ncr][2] per-cell bounds of masked candidates
   int n, roi, nr, nwpr, cw, nsheds, intervis;
   int cs, ncr, Q, nunits, G, upw, kpu, nbuckets, cap;
   // dynamic LDS layout (byte offsets)
@@ -225,11 +227,14 @@ struct SiteArgs {
   unsigned int spin_limit;  // barrier polls before the loop gives up
   int hash_fault;  // test knob: round whose pick hash the last workgroup
                    // alters (0: none), to exercise the divergence check
+  int iv_multi;    // intervis: rounds take several picks under cell bounds
+  int walk_evals;  // intervis: exact area computations per round's walk
+  int o_b, o_dcur, o_dprev, o_dbits;  // intervis LDS: bounds, dirty cells
 };
 
 // Device scratch SITE needs beyond SiteBuffers' arrays, and the size of the
 // double-buffered per-row key array (entries).
-size_t site_scratch_bytes(int nrows, int roi, int nsheds);
+size_t site_scratch_bytes(int nrows, int roi, int nsheds, int intervis);
 int site_key_rows(int nrows, int roi);
 
 // groups_cap: most workgroups the loop may use (0: half the CUs)

@@ -46,7 +46,7 @@ struct NoAllocLayout {
   size_t area, cum, rowmax, order, state, site, total;
 };
 
-NoAllocLayout noalloc_layout(int nrows, int roi, int nsheds, bool own_order) {
+NoAllocLayout noalloc_layout(int nrows, int roi, int nsheds, int intervis, bool own_order) {
   NoAllocLayout L{};
   size_t off = 0;
   auto put = [&](size_t &field, size_t bytes) {
@@ -59,7 +59,7 @@ NoAllocLayout noalloc_layout(int nrows, int roi, int nsheds, bool own_order) {
   put(L.rowmax, 2 * (size_t)site_key_rows(nrows, roi) * 8);
   put(L.order, own_order ? 4 * (size_t)nsheds : 0);
   put(L.state, sizeof(SiteState));
-  put(L.site, site_scratch_bytes(nrows, roi, nsheds));
+  put(L.site, site_scratch_bytes(nrows, roi, nsheds, intervis));
   L.total = off;
   return L;
 }
@@ -98,7 +98,7 @@ siting_status siting_site_it_noalloc(int nrows, int roi, int intervis, int nshed
       roi > kMaxRoi || nsheds < 1 || (intervis != 0 && intervis != 1))
     return SITING_EINVAL;
   set_limit_error(nullptr);
-  const NoAllocLayout L = noalloc_layout(nrows, roi, nsheds, order == nullptr);
+  const NoAllocLayout L = noalloc_layout(nrows, roi, nsheds, intervis, order == nullptr);
   if (L.total > kScratchBytes) {
     char msg[200];
     std::snprintf(msg, sizeof msg,

@@ -80,6 +80,9 @@ constexpr int kMaxBatch = kSiteMaxBatch;
 // position bit (bit 15: y < 32768) that flags a key whose successor in its
 // unit or lane list is unknown.
 constexpr int kPickWaves = 4;
+// Dirty cells a workgroup lists per round (intervis multi-pick); beyond
+// this it republishes every cell it owns.
+constexpr int kMaxDirtyCells = 1024;
 static_assert(kMaxBatch <= kWave, "lane j of the walk holds pick j");
 constexpr uint32_t kPickLast = 0x8000u;
 
@@ -521,6 +524,8 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   __shared__ uint32_t bw_pos[kMaxBatch];
   __shared__ int bw_n;
   __shared__ int bw_head;
+  __shared__ uint32_t ev_pos[kMaxBatch], ev_id[kMaxBatch];  // intervis walk: listed
+  __shared__ int ev_bd[kMaxBatch], ev_pick[kMaxBatch], ev_exact[kMaxBatch];
   __shared__ uint64_t xk[3 * (kPickWaves - 1)][kWave];  // pick lists of wavefronts 1..
   __shared__ uint32_t xp[3 * (kPickWaves - 1)][kWave];
 
@@ -531,7 +536,9 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   const int r2 = (2 * roi) * (2 * roi);
   const size_t S = (size_t)nr * nwpr;
   const int dw = nwpr + 1;
-  const int kmax = a.intervis ? 1 : kMaxBatch;
+  // intervis rounds of several picks (else one pick per round)
+  const bool ivm = a.intervis && a.iv_multi;
+  const int kmax = a.intervis && !ivm ? 1 : kMaxBatch;
   const bool hybrid = !a.intervis && a.tau_shift > 0;
 
   uint64_t *delta = (uint64_t *)(smem + a.o_delta);
@@ -553,6 +560,16 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   // a global bitmap, one word per 32 candidates from this group's start)
   unsigned int *seen = in_lds ? (unsigned int *)(smem + a.o_seen)
                               : a.sseen + (size_t)g * ((a.nsheds + 31) / 32 + 1);
+  // intervis multi-pick: Bd[i] bounds candidate i's area from above however
+  // the visibility mask turns (its new bits against cum, unmasked, when it
+  // was last computed: cum only grows), and the cells whose bounds changed
+  // this round and the last are listed for republishing
+  int32_t *Bd = !ivm ? nullptr : in_lds ? (int32_t *)(smem + a.o_b) : a.sB + B0;
+  int *dcur = ivm ? (int *)(smem + a.o_dcur) : nullptr;
+  int *dprev = ivm ? (int *)(smem + a.o_dprev) : nullptr;
+  unsigned int *dbits = ivm ? (unsigned int *)(smem + a.o_dbits) : nullptr;
+  __shared__ int dcur_n, dprev_n;
+  const int ncl = upw * kpu;  // owned cells, local index ul * kpu + k
   for (int i = tid; i < cnt; i += kLoopThreads) {
     const int c = a.sid[B0 + i];
     if (in_lds) {
@@ -561,6 +578,14 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     }
     T[i] = a.area[c];
     U[i] = 0;
+    if (ivm) Bd[i] = a.area[c];
+  }
+  if (ivm) {
+    for (int i = tid; i < (ncl + 31) / 32; i += kLoopThreads) dbits[i] = 0;
+    if (tid == 0) {
+      dcur_n = 0;
+      dprev_n = 0;
+    }
   }
   // start of cell k of owned unit ul, relative to B0 (k == kpu: unit end)
   const int32_t *bst = a.bstart + (size_t)g * upw * kpu;
@@ -595,6 +620,72 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       upos[2 * ul + 1] = t.p2;
     }
   };
+  // intervis multi-pick: the bound record of owned cell lc in buffer buf,
+  // over the cell's unused candidates whose last area was masked (U bit 2;
+  // only those can gain area, when a pick uncovers their cell -- the
+  // others' areas only shrink): (largest Bd << 32 | its position, second
+  // largest Bd << 32 | the largest's candidate)
+  auto cell_record = [&](int lc, int buf) {
+    const int ul = lc / kpu, k = lc - ul * kpu;
+    const int u = ul * G + g;
+    if (u >= nunits) return;
+    const int r = u / Q, cc = (u - r * Q) + k * Q;
+    if (cc >= a.ncr) return;
+    const int lo = OFF(ul, k), hi = OFF(ul, k + 1);
+    uint32_t b1 = 0, b2 = 0, p1 = 0, id1 = 0;
+    for (int i = lo; i < hi; i++) {
+      if ((U[i] & 5) != 4) continue;
+      const uint32_t b = (uint32_t)Bd[i];
+      if (b > b1) {
+        b2 = b1;
+        b1 = b;
+        p1 = P[i];
+        id1 = (uint32_t)C[i];
+      } else if (b > b2) {
+        b2 = b;
+      }
+    }
+    uint64_t *rec = a.cbnd + ((size_t)buf * a.ncr * a.ncr + (size_t)r * a.ncr + cc) * 2;
+    st_dev(&rec[0], ((uint64_t)b1 << 32) | p1);
+    st_dev(&rec[1], ((uint64_t)b2 << 32) | id1);
+  };
+  // lists owned candidate (cx, cy)'s cell as changed this round
+  auto mark_cell = [&](int cx, int cy) {
+    const int r = cx / a.cs, cc = cy / a.cs;
+    const int u = r * Q + cc % Q;
+    const int lc = (u / G) * kpu + cc / Q;
+    const unsigned int bit = 1u << (lc & 31);
+    if (!(atomicOr(&dbits[lc >> 5], bit) & bit)) {
+      const int at = atomicAdd(&dcur_n, 1);
+      if (at < kMaxDirtyCells) dcur[at] = lc;
+    }
+  };
+  // republishes, into buffer buf, the cells changed this round and the last
+  // (buffer buf last received them two rounds ago), then starts a new list
+  auto publish_cells = [&](int buf) {
+    __syncthreads();
+    const int nc = dcur_n, np = dprev_n;
+    if (nc > kMaxDirtyCells || np > kMaxDirtyCells) {
+      for (int lc = tid; lc < ncl; lc += kLoopThreads) cell_record(lc, buf);
+    } else {
+      for (int e = tid; e < nc + np; e += kLoopThreads)
+        cell_record(e < nc ? dcur[e] : dprev[e - nc], buf);
+    }
+    __syncthreads();
+    if (nc > kMaxDirtyCells) {
+      for (int i = tid; i < (ncl + 31) / 32; i += kLoopThreads) dbits[i] = 0;
+    } else {
+      for (int e = tid; e < nc; e += kLoopThreads) {
+        dbits[dcur[e] >> 5] = 0;
+        dprev[e] = dcur[e];
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      dprev_n = nc;
+      dcur_n = 0;
+    }
+  };
   auto publish = [&](int parity) {
     uint64_t *kn = a.keys + (size_t)parity * nunits * 2;
     uint32_t *pn = a.kpos + (size_t)parity * nunits * 2;
@@ -610,6 +701,11 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   for (int ul = wid; ul < upw; ul += kLoopWaves) unit_top2(ul);
   __syncthreads();
   publish(0);
+  if (ivm)
+    for (int lc = tid; lc < ncl; lc += kLoopThreads) {
+      cell_record(lc, 0);
+      cell_record(lc, 1);
+    }
   unsigned int bar_target = 0;
   const bool started = grid_sync(&a.st->barrier, bar_target, (unsigned)G, a.spin_limit);
 
@@ -689,6 +785,102 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         uint32_t pp = 0;
         int k = 0;
         int head = 0;  // area (or bound) of the largest key
+        // intervis: a masked candidate gains area only if a pick's
+        // viewshed covers its cell, so only masked candidates within roi of
+        // a pick count.  From the last round's records of the cells within
+        // roi of each pick taken: every cell's second largest masked bound
+        // (its largest, when the pick does not see that candidate's cell,
+        // stays masked) gives `bound`; a cell's largest, when the pick sees
+        // it, is listed (ev_*) and computed exactly against cum and the
+        // pick's bits only when it could beat the key at hand (its area
+        // then is at most that: other picks only cover more).  Picks are
+        // never masked (a masked area is 0).
+        uint32_t bound = 0;
+        int nwin = 0, ev_n = 0;
+        auto window_scan = [&](int j) {
+          const int px = __builtin_amdgcn_readlane((int)(pp >> 16), j);
+          const int py = __builtin_amdgcn_readlane((int)(pp & 0xffffu), j);
+          const uint32_t pid = ~(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)pk, j);
+          const uint64_t *psh = a.sheds + (size_t)pid * S;
+          const int r0 = max(0, px - roi) / a.cs, r1 = min(n - 1, px + roi) / a.cs;
+          const int c0 = max(0, py - roi) / a.cs, c1 = min(n - 1, py + roi) / a.cs;
+          const int ncw = c1 - c0 + 1, cnt_w = (r1 - r0 + 1) * ncw;
+          const uint64_t *cb = a.cbnd + (size_t)parity * a.ncr * a.ncr * 2;
+          uint32_t m = 0;
+          for (int e0 = 0; e0 < cnt_w; e0 += kWave) {
+            const int e = e0 + lane;
+            bool seen_by_pick = false;
+            uint64_t w0 = 0, w1 = 0;
+            if (e < cnt_w) {
+              const int r = r0 + e / ncw, cc = c0 + e % ncw;
+              const uint64_t *rec = cb + ((size_t)r * a.ncr + cc) * 2;
+              w0 = ld_dev(&rec[0]);
+              if ((w0 >> 32) != 0) {
+                w1 = ld_dev(&rec[1]);
+                const int dx = (int)((uint32_t)w0 >> 16) - px;
+                const int dy = (int)((uint32_t)w0 & 0xffffu) - py;
+                if (dx * dx + dy * dy <= roi * roi) {
+                  const int row = dx + roi, col = dy + roi;
+                  seen_by_pick = (psh[(size_t)row * nwpr + (col >> 6)] >> (63 - (col & 63))) & 1u;
+                }
+                const uint32_t b2 = (uint32_t)(w1 >> 32);
+                m = b2 > m ? b2 : m;
+              }
+            }
+            // list the seen ones (beyond the list, their bounds count)
+            const uint64_t sv = __ballot(seen_by_pick);
+            const int at = ev_n + (int)__builtin_amdgcn_mbcnt_hi(
+                                       (uint32_t)(sv >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)sv, 0));
+            if (seen_by_pick) {
+              if (at < kMaxBatch) {
+                ev_pos[at] = (uint32_t)w0;
+                ev_id[at] = (uint32_t)w1;
+                ev_bd[at] = (int)(w0 >> 32);
+                ev_pick[at] = j;
+                ev_exact[at] = -1;
+              } else {
+                const uint32_t b1 = (uint32_t)(w0 >> 32);
+                m = b1 > m ? b1 : m;
+              }
+            }
+            ev_n += __popcll(sv);
+          }
+          if (ev_n > kMaxBatch) ev_n = kMaxBatch;
+          const uint32_t wm = (uint32_t)wave_max_u64_dpp((uint64_t)m);
+          bound = wm > bound ? wm : bound;
+        };
+        // true if a listed candidate could reach area A (computed on demand,
+        // at most a.walk_evals per round: one wavefront computes them while
+        // the others wait)
+        int ev_evals = 0;
+        auto listed_block = [&](int A) -> bool {
+          for (int e = 0; e < ev_n; e++) {
+            if (ev_bd[e] < A) continue;
+            if (ev_exact[e] < 0) {
+              if (ev_evals >= a.walk_evals) return true;
+              ev_evals++;
+              const int j = ev_pick[e];
+              Winner Wj;
+              Wj.wx = __builtin_amdgcn_readlane((int)(pp >> 16), j);
+              Wj.wy = __builtin_amdgcn_readlane((int)(pp & 0xffffu), j);
+              Wj.w = (int)~(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)pk, j);
+              col_split(Wj.wy - roi, Wj.w0w, Wj.fbw);
+              Wj.nr = nr;
+              Wj.nwpr = nwpr;
+              Wj.dw = dw;
+              Wj.roi = roi;
+              Wj.sw = a.sheds + (size_t)Wj.w * S;
+              Wj.delta = nullptr;
+              const uint32_t ep = ev_pos[e];
+              const int s_exact = wave_new_bits(a.sheds + (size_t)ev_id[e] * S, (int)(ep >> 16),
+                                                (int)(ep & 0xffffu), a, Wj, lane, true);
+              if (lane == 0) ev_exact[e] = s_exact;
+              __builtin_amdgcn_wave_barrier();
+            }
+            if (ev_exact[e] >= A) return true;
+          }
+          return false;
+        };
         while (k < kmax) {
           const uint64_t bv = wave_max_u64_dpp(h.k[0]);
           if ((bv >> 32) == 0) break;     // no candidate adds area
@@ -705,6 +897,13 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
             near = ex * ex + ey * ey <= r2;
           }
           if (__any(near)) break;
+          if (ivm && k > 0) {
+            // a candidate near an earlier pick may gain area when the pick
+            // uncovers its cell (siting.cpp:452-457), up to its bound: this
+            // key must beat every such bound
+            for (; nwin < k; nwin++) window_scan(nwin);
+            if ((uint32_t)(bv >> 32) <= bound || listed_block((int)(bv >> 32))) break;
+          }
           if (lane == k) {
             pk = bv;
             pp = pos & ~kPickLast;
@@ -811,7 +1010,10 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         const int lo = OFF(ul, kc2), hi = OFF(ul, kc2 + 1);
         for (int j = lo + lane; j < hi; j += kWave)
           if (C[j] == w) U[j] = 1;
-        if (lane == 0) dirty[ul] = 1;
+        if (lane == 0) {
+          dirty[ul] = 1;
+          if (ivm) mark_cell(wx, wy);
+        }
       }
     }
 
@@ -830,7 +1032,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       W.sw = a.sheds + (size_t)w * S;
       W.delta = nullptr;
     }
-    if (a.intervis) {
+    if (a.intervis && k == 1) {
       // one pick: areas are recomputed against cum | delta while the union
       // is written (a reader sees either word, the OR is the same)
       if (a.delta_lds) {
@@ -891,6 +1093,10 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         failed = !grid_wait(&a.st->barrier, bar_target, a.spin_limit);
       }
     };
+    // intervis rounds of several picks recompute their update candidates
+    // against the whole union: its barrier completes first
+    const bool ivk = a.intervis && k > 1;
+    if (ivk) union_wait();
 
     // ---- update lists of picks 0..kc-1 among this workgroup's candidates.
     // Ranges: for each pick, the cells of owned units inside the pick's
@@ -964,16 +1170,19 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
             dirty[((cx / a.cs) * Q + (cy / a.cs) % Q) / G] = 1;
           } else if (m) {
             int acc = m;
-            if (!stop && a.intervis) {  // cpp:452-457 (one pick here)
+            if (!stop && a.intervis) {  // cpp:452-457
               uint64_t word = ld_dev(&a.cum[(size_t)cx * cw + (cy >> 6)]);
-              const int dr = cx - W.wx + roi, dk = (cy >> 6) - W.w0w;
-              if (dr >= 0 && dr < nr && dk >= 0 && dk <= nwpr) word |= W.word(dr, dk);
+              if (!ivk) {  // one pick: cum | its bits
+                const int dr = cx - W.wx + roi, dk = (cy >> 6) - W.w0w;
+                if (dr >= 0 && dr < nr && dk >= 0 && dk <= nwpr) word |= W.word(dr, dk);
+              }
               if (!((word >> (63 - (cy & 63))) & 1u)) acc = -m;
             }
             const int at = atomicAdd(&list_n, 1);
             list_i[at] = i;
             list_acc[at] = acc;
             dirty[((cx / a.cs) * Q + (cy / a.cs) % Q) / G] = 1;
+            if (ivm) mark_cell(cx, cy);
           }
         }
       }
@@ -997,7 +1206,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           const uint64_t *sh = a.sheds + (size_t)C[i] * S;
           int s;
           if (a.intervis)
-            s = wave_new_bits(sh, cx, cy, a, W, lane, true);
+            s = wave_new_bits(sh, cx, cy, a, W, lane, !ivk);
           else  // exact area minus the bits this round's picks covered
             s = T[i] - wave_lost_bits(sh, cx, cy, a, k, bw_pos, lane);
           if (lane == 0) list_acc[e] = s;
@@ -1006,6 +1215,16 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         if (tid < lcnt) {
           const int acc = list_acc[tid];
           T[list_i[tid]] = acc < 0 ? 0 : acc;
+          if (ivm) {
+            // masked: the bound stays and the candidate is flagged masked
+            const int i = list_i[tid];
+            if (acc >= 0) {
+              Bd[i] = acc;
+              U[i] &= (uint8_t)~4u;
+            } else {
+              U[i] |= 4;
+            }
+          }
         }
       }
       __syncthreads();
@@ -1075,6 +1294,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     for (int ul = tid; ul < upw; ul += kLoopThreads) dirty[ul] = 0;
     parity ^= 1;
     publish(parity);
+    if (ivm) publish_cells(parity);
     if (!grid_sync(&a.st->barrier, bar_target, (unsigned)G, a.spin_limit)) {
       niter = -1;
       break;
@@ -1154,6 +1374,7 @@ struct Plan {
   int cs, ncr, Q, nunits, G, upw, kpu, nbuckets, nchunks, cap, delta_lds, off_lds;
   size_t lds, nbwords;
   int o_delta, o_ukey, o_upos, o_dirty, o_off, o_c, o_pos, o_t, o_used, o_seen;
+  int iv, o_b, o_dcur, o_dprev, o_dbits;  // intervis multi-pick (o_*: -1 if off)
 };
 
 // Column classes per cell row.  More classes spread a pick's update window
@@ -1211,8 +1432,9 @@ long long stop_area(int n) {
 }
 
 Plan make_plan(int n, int roi, int nsheds, int groups_cap, int fixed_groups = 0,
-               int q_tuned = 0) {
+               int q_tuned = 0, int intervis = 0) {
   Plan p{};
+  p.iv = intervis;
   p.cs = cell_edge(n, roi);
   p.ncr = (n + p.cs - 1) / p.cs;
   p.Q = choose_q(n, roi, nsheds, q_tuned);
@@ -1249,9 +1471,20 @@ Plan make_plan(int n, int roi, int nsheds, int groups_cap, int fixed_groups = 0,
   p.off_lds = o + off_bytes <= kLdsBudget / 2;
   if (p.off_lds) o += off_bytes;
   o = align_up(o, 16);
+  p.o_b = p.o_dcur = p.o_dprev = p.o_dbits = -1;
+  if (intervis) {
+    // dirty cell lists of this round and the last, and a bit per owned cell
+    p.o_dcur = (int)o;
+    o += 4 * (size_t)kMaxDirtyCells;
+    p.o_dprev = (int)o;
+    o += 4 * (size_t)kMaxDirtyCells;
+    p.o_dbits = (int)o;
+    o = align_up(o + 4 * (((size_t)p.upw * p.kpu + 31) / 32), 16);
+  }
   const size_t fixed = o;
   const size_t room = kLdsBudget > fixed ? kLdsBudget - fixed : 0;
-  int cap = (int)(room / 14);  // id, pos, area, used, seen bit (rounded up)
+  // id, pos, area, used, seen bit (rounded up), and intervis area bounds
+  int cap = (int)(room / (intervis ? 18 : 14));
   cap = cap > 16 ? (cap - 16) / 16 * 16 : 0;
   p.cap = cap;
   p.o_c = (int)o;
@@ -1260,6 +1493,10 @@ Plan make_plan(int n, int roi, int nsheds, int groups_cap, int fixed_groups = 0,
   o += 4 * (size_t)cap;
   p.o_t = (int)o;
   o += 4 * (size_t)cap;
+  if (intervis) {
+    p.o_b = (int)o;
+    o += 4 * (size_t)cap;
+  }
   p.o_used = (int)o;
   o += (size_t)cap;
   o = align_up(o, 16);
@@ -1278,6 +1515,8 @@ struct Scratch {
   uint64_t *nb;
   unsigned int *sseen;
   uint64_t *phash;
+  int32_t *sB;
+  uint64_t *cbnd;
 };
 
 size_t scratch_layout(const Plan &p, int nsheds, uint8_t *base, Scratch *out) {
@@ -1304,6 +1543,10 @@ size_t scratch_layout(const Plan &p, int nsheds, uint8_t *base, Scratch *out) {
   s.keys = (uint64_t *)take(8 * 4 * (size_t)p.nunits);
   s.kpos = (uint32_t *)take(4 * 4 * (size_t)p.nunits);
   s.phash = (uint64_t *)take(8 * 2 * (size_t)kMaxGroups);
+  if (p.iv) {
+    s.sB = (int32_t *)take(4 * (size_t)nsheds);
+    s.cbnd = (uint64_t *)take(8 * 2 * 2 * (size_t)p.ncr * p.ncr);
+  }
   if (out) *out = s;
   return o;
 }
@@ -1314,11 +1557,11 @@ size_t scratch_layout(const Plan &p, int nsheds, uint8_t *base, Scratch *out) {
 // the job's tuning; the scratch size is taken for the largest group count,
 // which bounds every smaller one, and the largest over the built-in rule and
 // both tunable class counts.
-size_t site_scratch_bytes(int nrows, int roi, int nsheds) {
+size_t site_scratch_bytes(int nrows, int roi, int nsheds, int intervis) {
   size_t m = 0;
   for (int q : {0, 2, 3})
-    m = std::max(m, scratch_layout(make_plan(nrows, roi, nsheds, kMaxGroups, 0, q), nsheds,
-                                   nullptr, nullptr));
+    m = std::max(m, scratch_layout(make_plan(nrows, roi, nsheds, kMaxGroups, 0, q, intervis),
+                                   nsheds, nullptr, nullptr));
   return m;
 }
 
@@ -1328,7 +1571,7 @@ siting_status site_make_args(const Geometry &g, const int32_t *obs,
                              const uint64_t *sheds, const SiteBuffers &b, int slot,
                              SiteArgs *out, size_t *lds) {
   const int n = g.p.nrows, roi = g.p.roi, nsheds = g.d.nwanted;
-  const Plan p = make_plan(n, roi, nsheds, slot, slot, g.t.site_q);
+  const Plan p = make_plan(n, roi, nsheds, slot, slot, g.t.site_q, g.p.intervis);
   Scratch sc;
   scratch_layout(p, nsheds, b.scratch, &sc);
   SiteArgs a{};
@@ -1351,6 +1594,27 @@ siting_status site_make_args(const Geometry &g, const int32_t *obs,
   a.nb = sc.nb;
   a.sseen = sc.sseen;
   a.phash = sc.phash;
+  a.sB = sc.sB;
+  a.cbnd = sc.cbnd;
+  a.o_b = p.o_b;
+  a.o_dcur = p.o_dcur;
+  a.o_dprev = p.o_dprev;
+  a.o_dbits = p.o_dbits;
+  {
+    // intervis rounds of several picks (SITING_SITE_IV_MULTI=0: one per round)
+    const char *e = std::getenv("SITING_SITE_IV_MULTI");
+    a.iv_multi = g.p.intervis && !(e && *e && std::atoi(e) == 0);
+    // rule: 8 for roi up to 40 (measured: a 7594^2 roi-25 request's SITE
+    // 751 -> 651 ms), none beyond (a 7857^2 roi-50 one: 229 ms with none,
+    // 254-398 with 1-8); a job's tuning replaces the rule,
+    // SITING_SITE_WALK_EVALS overrides both
+    int ev = roi <= 40 ? 8 : 0;
+    if (g.t.site_walk_evals >= 1 && g.t.site_walk_evals <= kMaxBatch + 1)
+      ev = g.t.site_walk_evals - 1;
+    const char *w = std::getenv("SITING_SITE_WALK_EVALS");
+    if (w && *w) ev = std::max(0, std::min(kMaxBatch, std::atoi(w)));
+    a.walk_evals = ev;
+  }
   a.n = n;
   a.roi = roi;
   a.nr = g.d.nr;
@@ -1409,7 +1673,7 @@ siting_status site_launch_prep(const SiteArgs &a, hipStream_t s) {
 int site_loop_groups(const Geometry &g, int device_cus, int groups_cap) {
   int cap = device_cus / 2 > 0 ? device_cus / 2 : 1;
   if (groups_cap > 0 && groups_cap < cap) cap = groups_cap;
-  return make_plan(g.p.nrows, g.p.roi, g.d.nwanted, cap, 0, g.t.site_q).G;
+  return make_plan(g.p.nrows, g.p.roi, g.d.nwanted, cap, 0, g.t.site_q, g.p.intervis).G;
 }
 
 siting_status launch_site(const Geometry &g, const int32_t *obs,
@@ -1427,7 +1691,7 @@ siting_status launch_site(const Geometry &g, const int32_t *obs,
   if (groups_cap > 0 && groups_cap < cap) cap = groups_cap;
   if (const char *e = std::getenv("SITING_SITE_GROUPS"))  // tuning override
     if (std::atoi(e) > 0 && std::atoi(e) < cap) cap = std::atoi(e);
-  const Plan p = make_plan(g.p.nrows, g.p.roi, g.d.nwanted, cap, 0, g.t.site_q);
+  const Plan p = make_plan(g.p.nrows, g.p.roi, g.d.nwanted, cap, 0, g.t.site_q, g.p.intervis);
   SiteArgs a;
   size_t lds = 0;
   siting_status st = site_make_args(g, obs, sheds, b, p.G, &a, &lds);

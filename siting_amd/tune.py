@@ -12,14 +12,16 @@ own knobs, the others held at the rules:
   VIX       LDS halo 8, 16, 32
   VIEWSHED  threads per observer 128, 256, 512, times bitset mode (two LDS
             bitsets, one LDS bitset, HBM; the modes whose bitsets fit)
-  SITE      column classes 2, 3, times stale threshold shift 4, 5, 6
+  SITE      column classes 2, 3, times stale threshold shift 4, 5, 6;
+            and, on the same terrain with intervis 1, the pick walk's exact
+            area computations per round: none, 2 or 8
 
 A choice replaces the rule's only when it is at least 3 % faster.  The job
 is then run again with the chosen values, and its vix, observers, viewshed
 bitsets and selection must equal the rules' bit for bit, or the tuner stops.
 Each line of the cache: device key, roi class, targets class, edge class,
-halo, threads, mode, column classes, shift, then the measured stage times
-as a comment.
+halo, threads, mode, column classes, shift, 1 + walk computations, then the
+measured stage times as a comment.
 """
 from __future__ import annotations
 
@@ -89,7 +91,8 @@ def rule_choice(roi: int, ntests: int, n: int, nsheds: int) -> dict:
     q_list = 3.14159265358979 * 4.0 * roi * roi * nsheds / (n * n)
     return {"vix_halo": 32 if roi >= 64 and ntests >= 64 else 16,
             "viewshed_threads": 256, "viewshed_mode": modes[0],
-            "site_q": 3 if q_list >= 3000.0 else 2, "site_tau_shift": 5}
+            "site_q": 3 if q_list >= 3000.0 else 2, "site_tau_shift": 5,
+            "site_walk_evals": 9 if roi <= 40 else 1}
 
 
 def tune_class(key: str, roi: int, ntests: int, edge: int, reps: int, log):
@@ -147,6 +150,26 @@ def tune_class(key: str, roi: int, ntests: int, edge: int, reps: int, log):
         job.sync()
         if _outputs(job) != ref:
             raise SystemExit(f"tuned run differs from the rules' for {w}")
+    # intervis: the pick walk's exact computations, on the same terrain
+    pi = sa.make_params(n, roi, 100, ntests, 16, n * n // 256, 1)
+    with sa.Job(pi) as job:
+        job.upload_dem(dem)
+        job.set_tuning(sa.Tuning(**choice))
+        job.run()
+        job.sync()
+        ref_iv = _outputs(job)
+        te = {}
+        for ev in (1, 3, 9):
+            job.set_tuning(sa.Tuning(**dict(choice, site_walk_evals=ev)))
+            te[ev] = _time_stage(job, "site", max(1, reps - 1))
+        choice["site_walk_evals"] = _pick(te, rule["site_walk_evals"])
+        times["site_iv"] = te[choice["site_walk_evals"]]
+        rule_ms["site_iv"] = te[rule["site_walk_evals"]]
+        job.set_tuning(sa.Tuning(**choice))
+        job.site()
+        job.sync()
+        if _outputs(job) != ref_iv:
+            raise SystemExit(f"tuned intervis run differs from the rules' for {w}")
     log(f"{key} roi {roi} targets {ntests} edge {edge}: {choice} "
         f"ms {times} (rules {rule_ms})")
     return choice, times, rule_ms
@@ -167,7 +190,8 @@ def main(argv=None):
 
     lines = ["# siting tuning cache, written by python -m siting_amd.tune",
              "# key roi targets edge vix_halo viewshed_threads viewshed_mode site_q "
-             "site_tau_shift  # tuned ms (rules' ms): vix viewshed site"]
+             "site_tau_shift site_walk_evals  # tuned ms (rules' ms): vix viewshed site "
+             "site_intervis"]
     t0 = time.time()
     for roi in [int(x) for x in args.rois.split(",")]:
         for ntests in NTESTS:
@@ -175,9 +199,11 @@ def main(argv=None):
                 c, t, r = tune_class(key, roi, ntests, edge, args.reps, log)
                 lines.append(
                     f"{key} {roi} {ntests} {edge} {c['vix_halo']} {c['viewshed_threads']} "
-                    f"{c['viewshed_mode']} {c['site_q']} {c['site_tau_shift']}  # "
-                    f"{t['vix']:.3f} {t['viewshed']:.3f} {t['site']:.3f} "
-                    f"({r['vix']:.3f} {r['viewshed']:.3f} {r['site']:.3f})")
+                    f"{c['viewshed_mode']} {c['site_q']} {c['site_tau_shift']} "
+                    f"{c['site_walk_evals']}  # "
+                    f"{t['vix']:.3f} {t['viewshed']:.3f} {t['site']:.3f} {t['site_iv']:.3f} "
+                    f"({r['vix']:.3f} {r['viewshed']:.3f} {r['site']:.3f} "
+                    f"{r['site_iv']:.3f})")
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     with open(args.out, "w") as f:
         f.write("\n".join(lines) + "\n")

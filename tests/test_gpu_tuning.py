@@ -13,14 +13,17 @@ pytestmark = pytest.mark.gpu
 # (n, roi, ntests, bw, nwanted0, intervis, rough_h, seed)
 CASES = [(300, 40, 64, 10, 900, 0, 0.8, 91),
          (260, 100, 32, 16, 264, 1, 0.6, 92),
-         (240, 150, 32, 16, 225, 0, 0.7, 93)]
+         (240, 150, 32, 16, 225, 0, 0.7, 93),
+         (400, 25, 64, 8, 2500, 1, 0.6, 94)]
 
 CHOICES = [dict(vix_halo=8), dict(vix_halo=16), dict(vix_halo=32),
            dict(viewshed_threads=128), dict(viewshed_threads=512),
            dict(viewshed_mode=1), dict(viewshed_mode=2), dict(viewshed_mode=3),
            dict(site_q=2), dict(site_q=3), dict(site_tau_shift=1),
-           dict(site_tau_shift=9), dict(vix_halo=32, viewshed_threads=128,
-                                        viewshed_mode=2, site_q=3, site_tau_shift=4)]
+           dict(site_tau_shift=9), dict(site_walk_evals=1), dict(site_walk_evals=3),
+           dict(site_walk_evals=33),
+           dict(vix_halo=32, viewshed_threads=128, viewshed_mode=2, site_q=3,
+                site_tau_shift=4, site_walk_evals=9)]
 
 
 def _outputs(job):
@@ -67,7 +70,7 @@ def test_job_takes_the_cache_entry(gpu, tmp_path, monkeypatch):
     with gpu.Job(p) as job:
         assert job.get_tuning().as_dict() == {"vix_halo": 8, "viewshed_threads": 128,
                                               "viewshed_mode": 3, "site_q": 3,
-                                              "site_tau_shift": 6}
+                                              "site_tau_shift": 6, "site_walk_evals": 0}
         job.upload_dem(dem)
         job.run()
         job.sync()

@@ -12,7 +12,7 @@ def _write(tmp_path, body):
 
 CACHE = """# comment line
 gfx950:256 100 64 4096 32 512 2 3 4  # trailing comment
-gfx950:256 25 32 1024 8 128 3 2 6
+gfx950:256 25 32 1024 8 128 3 2 6 9
 gfx950:256 200 64 2048 16 256 1 2 5
 gfx942:304 100 64 4096 16 256 1 2 5
 bad line with words
@@ -25,11 +25,11 @@ def test_lookup_classes(sa, tmp_path, monkeypatch):
     # roi 110 -> class 100, 64 targets, edge 4000 -> class 4096
     t = sa.tuning_lookup("gfx950:256", sa.make_params(4000, 110, 100, 64, 16, 1000, 0))
     assert t.as_dict() == {"vix_halo": 32, "viewshed_threads": 512, "viewshed_mode": 2,
-                           "site_q": 3, "site_tau_shift": 4}
+                           "site_q": 3, "site_tau_shift": 4, "site_walk_evals": 0}
     # roi 30 -> 25, 40 targets -> 32, edge 1500 -> 1024
     t = sa.tuning_lookup("gfx950:256", sa.make_params(1500, 30, 100, 40, 16, 1000, 0))
-    assert (t.vix_halo, t.viewshed_threads, t.viewshed_mode, t.site_q, t.site_tau_shift) == \
-        (8, 128, 3, 2, 6)
+    assert (t.vix_halo, t.viewshed_threads, t.viewshed_mode, t.site_q, t.site_tau_shift,
+            t.site_walk_evals) == (8, 128, 3, 2, 6, 9)
     # roi 250 -> 200, edge 3000 -> 2048
     t = sa.tuning_lookup("gfx950:256", sa.make_params(3000, 250, 100, 64, 16, 1000, 1))
     assert t.viewshed_mode == 1
@@ -74,12 +74,14 @@ def test_committed_cache_parses(sa, monkeypatch):
     rows = [r for r in rows if r]
     assert rows
     for r in rows:
-        assert len(r) == 9, r
-        roi, nt, edge, halo, th, mode, q, s = map(int, r[1:])
+        assert len(r) in (9, 10), r
+        roi, nt, edge, halo, th, mode, q, s = map(int, r[1:9])
+        ev = int(r[9]) if len(r) == 10 else 0
         assert roi in (25, 50, 100, 150, 200) and nt in (32, 64) and edge in (1024, 2048, 4096)
         assert halo in (8, 16, 32) and th % 64 == 0 and mode in (1, 2, 3)
-        assert q in (2, 3) and 1 <= s <= 20
+        assert q in (2, 3) and 1 <= s <= 20 and 0 <= ev <= 33
         key = r[0]
         got = sa.tuning_lookup(key, sa.make_params(edge, roi, 100, nt, 16, 1000, 0))
         assert got.as_dict() == {"vix_halo": halo, "viewshed_threads": th,
-                                 "viewshed_mode": mode, "site_q": q, "site_tau_shift": s}
+                                 "viewshed_mode": mode, "site_q": q, "site_tau_shift": s,
+                                 "site_walk_evals": ev}
