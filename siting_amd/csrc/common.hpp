@@ -229,6 +229,7 @@ struct SiteArgs {
                    // alters (0: none), to exercise the divergence check
   int iv_multi;    // intervis: rounds take several picks under cell bounds
   int walk_evals;  // intervis: exact area computations per round's walk
+  int near_evals;  // intervis 0: exact area computations per round's walk
   int o_b, o_dcur, o_dprev, o_dbits;  // intervis LDS: bounds, dirty cells
 };
 

@@ -26,12 +26,20 @@
 //                    in a global slice of the same layout) and the two best
 //                    keys of each of its units.  A round:
 //                    1. every workgroup stages all published unit keys and
-//                       picks, identically, the longest prefix of the global
-//                       key order (at most 16 picks; 1 under intervis) whose
-//                       members are pairwise more than 2*roi apart.  None of
-//                       them is in another's update list, so the sequential
-//                       loop would pick exactly these, in this order, in as
-//                       many iterations (areas never grow when intervis is 0).
+//                       walks, identically, down the global key order (at
+//                       most 32 picks).  A key more than 2*roi from every
+//                       earlier pick of the round is exact and is taken.
+//                       Intervis 0: a nearer key is re-keyed to its exact
+//                       area after those picks (computed by the whole
+//                       workgroup) and goes back into the order, as the
+//                       sequential loop recomputed it after each pick.
+//                       Intervis 1: a masked candidate may gain area when a
+//                       pick uncovers its cell, so a key is taken only if it
+//                       beats every such candidate's bound (per-cell records
+//                       of the masked bounds, the largest computed exactly
+//                       on demand); a nearer key ends the round.  A walk
+//                       that read cum is followed by a grid barrier before
+//                       step 3 writes it.
 //                    2. counters advance pick by pick; the 95% rule may end
 //                       the round after any pick.
 //                    3. the picks' sheds, shifted to terrain words, are ORed
@@ -40,8 +48,10 @@
 //                    4. every workgroup walks the candidates of its units that
 //                       lie within 2*roi of a pick (cell ranges), each once,
 //                       and lowers its area by the pick bits it loses
-//                       (intervis 0), or recomputes it against cum | delta
-//                       (intervis 1, one pick, no barrier in step 3).
+//                       (intervis 0; one below a threshold keeps its old area
+//                       as a stale bound, re-evaluated once the round's head
+//                       falls to it), or recomputes it against cum | delta
+//                       (intervis 1; one pick needs no barrier in step 3).
 //                    5. dirty units re-rank their top two, publish them into
 //                       the other half of a double-buffered key array, grid
 //                       barrier.
@@ -83,6 +93,8 @@ constexpr int kPickWaves = 4;
 // Dirty cells a workgroup lists per round (intervis multi-pick); beyond
 // this it republishes every cell it owns.
 constexpr int kMaxDirtyCells = 1024;
+// Candidates the pick walk may re-key in a round (intervis 0).
+constexpr int kMaxRekey = 8;
 static_assert(kMaxBatch <= kWave, "lane j of the walk holds pick j");
 constexpr uint32_t kPickLast = 0x8000u;
 
@@ -423,6 +435,66 @@ __device__ __forceinline__ int wave_lost_bits(const uint64_t *sh, int cx, int cy
   return wave_sum_i32(s);
 }
 
+// New bits of candidate (cx, cy)'s shed against cum | the sheds of np
+// picks (positions ppos[], shed indices pid[], in LDS), i.e. its area once
+// they are taken (intervis 0) or, for one pick, its area bound after that
+// pick (intervis 1): lane gl of gs lanes' share.  Items are (shed row,
+// terrain word) pairs as in lane_new_bits.
+__device__ __forceinline__ int lane_new_bits_picks(const uint64_t *csh, int cx, int cy, int np,
+                                                   const uint32_t *ppos, const uint32_t *pid,
+                                                   const SiteArgs &a, int gl, int gs) {
+  const int roi = a.roi, nwpr = a.nwpr, cw = a.cw, n = a.n, nr = 2 * roi + 1;
+  const size_t S = (size_t)nr * nwpr;
+  int w0c, fbc;
+  col_split(cy - roi, w0c, fbc);
+  const int klo = max(w0c, 0) - w0c;
+  const int khi = min((cy + roi) >> 6, cw - 1) - w0c;
+  const int nk = khi - klo + 1;
+  const int rlo = max(0, cx - roi) - (cx - roi);
+  const int rhi = min(n - 1, cx + roi) - (cx - roi);
+  const int items = (rhi - rlo + 1) * nk;
+  const float inv_nk = 1.0f / (float)nk;
+  const bool fdiv = items < (1 << 21);
+  int s = 0;
+  for (int base = 0; base < items; base += gs * kBatch) {
+    uint64_t cv[kBatch], mv[kBatch];
+    int xr[kBatch], jw[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
+      const int it = base + u * gs + gl;
+      const bool ok = it < items;
+      const int q = fdiv ? (int)(((float)it + 0.5f) * inv_nk) : it / nk;
+      const int row = rlo + q, k = klo + (it - q * nk);
+      const uint64_t *srow = csh + (size_t)row * nwpr;
+      const uint64_t cur = (ok && k < nwpr) ? srow[k] : 0;
+      const uint64_t prv = (ok && k > 0) ? srow[k - 1] : 0;
+      uint64_t c = cur >> fbc;
+      if (fbc != 0) c |= prv << (64 - fbc);
+      xr[u] = cx - roi + row;
+      jw[u] = w0c + k;
+      cv[u] = c;
+      // (no load waits on another: one memory latency per batch)
+      mv[u] = ok ? ld_dev(&a.cum[(size_t)xr[u] * cw + jw[u]]) : 0;
+    }
+    for (int j = 0; j < np; j++) {
+      const uint32_t pq = ppos[j];
+      const int px = (int)(pq >> 16), py = (int)(pq & 0xffffu);
+      const uint64_t *psh = a.sheds + (size_t)pid[j] * S;
+      int w0p, fbp;
+      col_split(py - roi, w0p, fbp);
+#pragma unroll
+      for (int u = 0; u < kBatch; u++) {
+        const int dr = xr[u] - px + roi, dk = jw[u] - w0p;
+        if (dr >= 0 && dr < nr && dk >= 0 && dk <= nwpr)
+          mv[u] |= winner_word(psh, nwpr, fbp, dr, dk);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) s += __popcll(cv[u] & ~mv[u]);
+  }
+  return s;
+}
+
 // Maximum of a 64-bit value over the wavefront, in every lane, with DPP row
 // operations (no LDS traffic): quad swaps, half-row and row mirrors, then the
 // row broadcasts that fold rows 0-3 into lane 63.
@@ -524,6 +596,21 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   __shared__ uint32_t bw_pos[kMaxBatch];
   __shared__ int bw_n;
   __shared__ int bw_head;
+  __shared__ uint64_t r_key[kMaxRekey];  // the walk's rekeyed candidates
+  __shared__ uint32_t r_pos[kMaxRekey];
+  __shared__ int r_mark[kMaxRekey];
+  __shared__ int bw_npair[kWave];  // earlier picks of the round within 2*roi of pick i
+  __shared__ int bw_readcum;       // the walk read cum
+  // The walk's area computations are shared by the whole workgroup: the
+  // walking wavefront posts a request (candidate, picks) and takes its
+  // share, the others, waiting in a service loop, take theirs and add them
+  // to mb_acc with their arrival (arrivals << 40 | sum).  mb_seq numbers
+  // the requests (-1: the walk is over).
+  __shared__ int mb_seq;
+  __shared__ unsigned long long mb_acc;
+  __shared__ uint32_t mb_cid, mb_cpos;
+  __shared__ int mb_np;
+  __shared__ uint32_t mb_ppos[kMaxBatch], mb_pid[kMaxBatch];
   __shared__ uint32_t ev_pos[kMaxBatch], ev_id[kMaxBatch];  // intervis walk: listed
   __shared__ int ev_bd[kMaxBatch], ev_pick[kMaxBatch], ev_exact[kMaxBatch];
   __shared__ uint64_t xk[3 * (kPickWaves - 1)][kWave];  // pick lists of wavefronts 1..
@@ -724,7 +811,13 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   if (tid == 0) ph_round = 0;
   __syncthreads();
 
+  // the walk's helpers (see mb_seq): rounds that may compute areas
+  // (a shed box of at most kWave * kBatch items is computed by the walking
+  // wavefront alone: the handoff would cost more than it saves)
+  const bool walk_help = (a.intervis ? (ivm && a.walk_evals > 0) : a.near_evals > 0) &&
+                         (2 * roi + 1) * (nwpr + 1) > kWave * kBatch;
   while (started) {
+    if (tid == 0) mb_seq = 0;
     if (wid == kLoopWaves - 1) {
       bool bad = false;
       if (ph_round > 0) {
@@ -736,12 +829,14 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       if (lane == 0) ph_bad = any_bad;
     }
     // ---- pick the round's winners (identical in every workgroup).  Winners
-    // are a prefix of the descending order of all published unit keys in
-    // which no two are within 2*roi of each other: none of them is in
-    // another's update list, so the sequential loop would pick them in this
-    // order (areas only shrink when intervis is 0).  A unit's second key
-    // sorts below its first, so the order needs no per-unit state; taking a
-    // unit's second key ends the prefix, since its third is unknown.
+    // follow the descending order of all published unit keys, as the
+    // sequential loop would pick them: a key far (> 2*roi) from every
+    // earlier pick of the round is exact; a nearer one is re-keyed to its
+    // exact area after those picks (intervis 0; areas only shrink, so it
+    // goes back into the order) or ends the round (intervis 1, unless the
+    // cell bounds admit it).  A unit's second key sorts below its first, so
+    // the order needs no per-unit state; taking a unit's second key ends
+    // the round, since its third is unknown.
     //
     // Lane l of wavefronts 0..3 keeps the three largest keys of units
     // l, l + 64, ... (strided by 256) in registers; wavefronts 1..3 hand
@@ -795,6 +890,51 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         // pick's bits only when it could beat the key at hand (its area
         // then is at most that: other picks only cover more).  Picks are
         // never masked (a masked area is 0).
+        // Area of candidate cid (at cpos) against cum | the picks of the
+        // lanes in `picks`, computed with the helpers.
+        int mb_n = 0;
+        auto walk_area = [&](uint32_t cid, uint32_t cpos, uint64_t picks) -> int {
+          if (!walk_help) {  // a small shed: this wavefront alone
+            const int np = __popcll(picks);
+            const int slot = (int)__builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(picks >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)picks, 0));
+            if ((picks >> lane) & 1) {
+              mb_ppos[slot] = pp;
+              mb_pid[slot] = ~(uint32_t)pk;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            return wave_sum_i32(lane_new_bits_picks(a.sheds + (size_t)cid * S, (int)(cpos >> 16),
+                                                    (int)(cpos & 0xffffu), np, mb_ppos, mb_pid, a,
+                                                    lane, kWave));
+          }
+          const int slot = (int)__builtin_amdgcn_mbcnt_hi(
+              (uint32_t)(picks >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)picks, 0));
+          if ((picks >> lane) & 1) {
+            mb_ppos[slot] = pp;
+            mb_pid[slot] = ~(uint32_t)pk;
+          }
+          if (lane == 0) {
+            mb_cid = cid;
+            mb_cpos = cpos;
+            mb_np = __popcll(picks);
+            mb_acc = 0;
+          }
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+          mb_n++;
+          if (lane == 0) __hip_atomic_store(&mb_seq, mb_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+          const int own = wave_sum_i32(lane_new_bits_picks(
+              a.sheds + (size_t)cid * S, (int)(cpos >> 16), (int)(cpos & 0xffffu),
+              __popcll(picks), mb_ppos, mb_pid, a, lane, kLoopThreads));
+          unsigned long long acc = 0;
+          if (lane == 0)
+            while (((acc = __hip_atomic_load(&mb_acc, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) >> 40) <
+                   (unsigned long long)(kLoopWaves - 1))
+              __builtin_amdgcn_s_sleep(1);
+          acc = __shfl(acc, 0, kWave);
+          return own + (int)(acc & ((1ull << 40) - 1));
+        };
         uint32_t bound = 0;
         int nwin = 0, ev_n = 0;
         auto window_scan = [&](int j) {
@@ -859,21 +999,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
             if (ev_exact[e] < 0) {
               if (ev_evals >= a.walk_evals) return true;
               ev_evals++;
-              const int j = ev_pick[e];
-              Winner Wj;
-              Wj.wx = __builtin_amdgcn_readlane((int)(pp >> 16), j);
-              Wj.wy = __builtin_amdgcn_readlane((int)(pp & 0xffffu), j);
-              Wj.w = (int)~(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)pk, j);
-              col_split(Wj.wy - roi, Wj.w0w, Wj.fbw);
-              Wj.nr = nr;
-              Wj.nwpr = nwpr;
-              Wj.dw = dw;
-              Wj.roi = roi;
-              Wj.sw = a.sheds + (size_t)Wj.w * S;
-              Wj.delta = nullptr;
-              const uint32_t ep = ev_pos[e];
-              const int s_exact = wave_new_bits(a.sheds + (size_t)ev_id[e] * S, (int)(ep >> 16),
-                                                (int)(ep & 0xffffu), a, Wj, lane, true);
+              const int s_exact = walk_area(ev_id[e], ev_pos[e], 1ull << ev_pick[e]);
               if (lane == 0) ev_exact[e] = s_exact;
               __builtin_amdgcn_wave_barrier();
             }
@@ -881,8 +1007,62 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           }
           return false;
         };
+        // intervis 0: a key within 2*roi of an earlier pick of the round is
+        // re-keyed -- its exact area once the earlier picks within 2*roi
+        // are taken, as the sequential loop recomputed it after each of
+        // them -- and joins the rekeyed set (r_*, exact through the first
+        // r_mark picks) instead of ending the round; the largest of the
+        // lanes' keys and the rekeyed set goes next (areas only shrink, so a
+        // lane's key bounds the area of its candidate).  At most
+        // a.near_evals such computations per round.
+        int rn = 0, nevals = 0, npair = 0;
         while (k < kmax) {
           const uint64_t bv = wave_max_u64_dpp(h.k[0]);
+          if (rn > 0) {
+            const uint64_t rkv = lane < rn ? r_key[lane] : 0;
+            const uint64_t rv = wave_max_u64_dpp(rkv);
+            // (equal only to the bound a rekeyed lane left, which is its own)
+            if (rv >= bv) {
+              const int e = (int)__builtin_ctzll(__ballot(rkv == rv));
+              const uint32_t rpos = r_pos[e];
+              const int mark = r_mark[e];
+              const int rx = (int)(rpos >> 16), ry = (int)(rpos & 0xffffu);
+              bool fresh = false, near_any = false;
+              if (lane < k) {
+                const int ex = rx - (int)(pp >> 16), ey = ry - (int)(pp & 0xffffu);
+                near_any = ex * ex + ey * ey <= r2;
+                fresh = near_any && lane >= mark;
+              }
+              const uint64_t nearp = __ballot(near_any);
+              if (__ballot(fresh)) {  // picks since its last computation reach it
+                if (nevals >= a.near_evals) break;
+                nevals++;
+                const int area = walk_area(~(uint32_t)rv, rpos, nearp);
+                if (lane == 0) {
+                  r_key[e] = ((uint64_t)(uint32_t)area << 32) | (uint32_t)rv;
+                  r_mark[e] = k;
+                }
+                __builtin_amdgcn_wave_barrier();
+                continue;
+              }
+              if ((rv >> 32) == 0) break;  // no candidate adds area
+              // take it; it was in the update list of each earlier pick
+              // within 2*roi (cpp:540-544), which leaves every pick out
+              if (lane == k) {
+                pk = rv;
+                pp = rpos;
+                npair = __popcll(nearp);
+              }
+              if (lane == 0) {
+                r_key[e] = r_key[rn - 1];
+                r_pos[e] = r_pos[rn - 1];
+                r_mark[e] = r_mark[rn - 1];
+              }
+              rn--;
+              k++;
+              continue;
+            }
+          }
           if ((bv >> 32) == 0) break;     // no candidate adds area
           if (k == 0) head = (int)(bv >> 32);
           // the lane that holds it (keys are distinct)
@@ -896,7 +1076,36 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
             const int ey = py - (int)(pp & 0xffffu);
             near = ex * ex + ey * ey <= r2;
           }
-          if (__any(near)) break;
+          const uint64_t nearp = __ballot(near);
+          if (nearp) {
+            // intervis: a near candidate may gain area (the mask lifts), so
+            // the round ends
+            if (a.intervis || rn == kMaxRekey || nevals >= a.near_evals) break;
+            nevals++;
+            const int area = walk_area(~(uint32_t)bv, pos & ~kPickLast, nearp);
+            if (lane == 0) {
+              r_key[rn] = ((uint64_t)(uint32_t)area << 32) | (uint32_t)bv;
+              r_pos[rn] = pos & ~kPickLast;
+              r_mark[rn] = k;
+            }
+            __builtin_amdgcn_wave_barrier();
+            rn++;
+            if (lane == src) {
+              if (pos & kPickLast) {
+                // the lane's next keys are unknown but below this one: it
+                // stays as a bound that ends the walk if it comes first
+                h.p[0] = pos | 0x80000000u;
+              } else {
+                h.k[0] = h.k[1];
+                h.p[0] = h.p[1];
+                h.k[1] = h.k[2];
+                h.p[1] = h.p[2];
+                h.k[2] = 0;
+                h.p[2] = 0;
+              }
+            }
+            continue;
+          }
           if (ivm && k > 0) {
             // a candidate near an earlier pick may gain area when the pick
             // uncovers its cell (siting.cpp:452-457), up to its bound: this
@@ -922,10 +1131,30 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         if (lane < k) {
           bw_key[lane] = pk;
           bw_pos[lane] = pp;
+          bw_npair[lane] = npair;
         }
         if (lane == 0) {
           bw_n = k;
           bw_head = head;
+          bw_readcum = nevals + ev_evals > 0;
+          if (walk_help) __hip_atomic_store(&mb_seq, -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      } else if (walk_help) {
+        // the walk's helpers: a share of each area computation it posts
+        int seen = 0;
+        for (;;) {
+          int q;
+          while ((q = __hip_atomic_load(&mb_seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) == seen)
+            __builtin_amdgcn_s_sleep(1);
+          if (q < 0) break;
+          seen = q;
+          const uint32_t cpos = mb_cpos;
+          const int part = wave_sum_i32(lane_new_bits_picks(
+              a.sheds + (size_t)mb_cid * S, (int)(cpos >> 16), (int)(cpos & 0xffffu), mb_np,
+              mb_ppos, mb_pid, a, wid * kWave + lane, kLoopThreads));
+          if (lane == 0)
+            __hip_atomic_fetch_add(&mb_acc, (1ull << 40) + (unsigned long long)part, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
         }
       }
     }
@@ -938,6 +1167,12 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     // no pick while the largest key is stale: the round only re-evaluates
     const bool eval_only = k == 0 && bw_head > 0;
     if (k == 0 && !eval_only) break;  // cpp:589-592
+    // A walk that read cum (exact areas) must end in every workgroup before
+    // any of them ORs this round's picks into it.  Every workgroup's walk is
+    // the same up to its first read of cum, so either all arrive here or
+    // none does.
+    const bool walk_sync = bw_readcum;
+    if (walk_sync) grid_arrive(&a.st->barrier, bar_target, (unsigned)G);
     if (wid == kLoopWaves - 1) {
       // hash of (picks, count, head); stored before this round's barriers
       uint64_t v = 0;
@@ -997,6 +1232,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         const int w = (int)(~(uint32_t)bw_key[tid]);
         a.used[w] = 1;
         a.order[nused0 + tid] = w;
+        my_u += bw_npair[tid];  // in the update lists of earlier picks
       }
     }
     // owners mark the picks used: wavefront i takes picks i, i + 16, ...
@@ -1015,6 +1251,11 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           if (ivm) mark_cell(wx, wy);
         }
       }
+    }
+
+    if (walk_sync && !grid_wait(&a.st->barrier, bar_target, a.spin_limit)) {
+      niter = -1;
+      break;
     }
 
     // ---- union of the picks into cum (cpp:393-419)
@@ -1055,8 +1296,9 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       }
     } else if (k > 0) {
       // every pick: OR its shifted shed into cum and keep the bits that were
-      // new (the pick's area; picks are > 2*roi apart so their sheds share
-      // no bit, only, possibly, an edge word -- hence the atomic)
+      // new; picks within 2*roi of each other share bits, and the atomic
+      // gives each new bit to exactly one of them, so the kept bits of the
+      // picks are disjoint and together the union's new bits
       __syncthreads();
       const int total = k * nr * dw;
       const int chunk = (total + G - 1) / G;
@@ -1614,6 +1856,13 @@ siting_status site_make_args(const Geometry &g, const int32_t *obs,
     const char *w = std::getenv("SITING_SITE_WALK_EVALS");
     if (w && *w) ev = std::max(0, std::min(kMaxBatch, std::atoi(w)));
     a.walk_evals = ev;
+    // intervis 0: 8 rekeys per round (measured on roi 25-150 jobs of 894^2 to
+    // 4096^2: 10-33 % less SITE time, 8 and 16 alike, 2 to 4 behind);
+    // SITING_SITE_NEAR_EVALS overrides
+    int ne = 8;
+    const char *ne_env = std::getenv("SITING_SITE_NEAR_EVALS");
+    if (ne_env && *ne_env) ne = std::max(0, std::min(kMaxBatch, std::atoi(ne_env)));
+    a.near_evals = g.p.intervis ? 0 : ne;
   }
   a.n = n;
   a.roi = roi;
