@@ -253,6 +253,14 @@ def run_batch(sa, np, args):
 # single jobs
 # ---------------------------------------------------------------------------
 
+def tuning_report(sa, params, used):
+    """The kernel shape choices a job ran with, and whether they came from
+    the tuning cache (zeros: the built-in rules)."""
+    cached = sa.tuning_lookup(sa.device_key(0), params)
+    return {"from_cache": cached is not None and cached.as_dict() == used.as_dict(),
+            **used.as_dict()}
+
+
 def run_single(sa, np, w, steps, digest=None, stages=False):
     n = w["nrows"]
     dem = make_dem(sa, w)
@@ -274,7 +282,8 @@ def run_single(sa, np, w, steps, digest=None, stages=False):
     t = (time.perf_counter() - t0) / steps
     res = {"points_per_s": round(n * n / t, 1), "ms_per_step": round(1000 * t, 3),
            "site": {"nused": st.nused, "niter": st.niter, "sum_u": st.sum_u,
-                    "coverage": st.coverage}}
+                    "rounds": st.rounds, "coverage": st.coverage},
+           "tuning": tuning_report(sa, job.params, job.get_tuning())}
     if stages:
         abytes = algorithmic_bytes(n, d, st)
         res["stages"] = {}
@@ -508,7 +517,13 @@ def main():
                    "batch_equals_plain": bt["same"],
                    "batch_device_ms": round(bt["dev_ms"], 3),
                    "site_queue_ms": round(bt["site_ms"], 3),
-                   "graph": not args.no_graph},
+                   "graph": not args.no_graph,
+                   "tuning_cache": os.path.relpath(sa.tuning_cache_path(), ROOT)
+                   if sa.tuning_cache_path() else None,
+                   "device_key": sa.device_key(0),
+                   "batch_jobs_tuned_from_cache": sum(
+                       sa.tuning_lookup(sa.device_key(0), params_of(sa, w)) is not None
+                       for w in bt["batch"])},
         "single_jobs": singles,
         "serve": serve,
         "p99_job_latency_s": serve["p99_latency_s"] if serve else None,
