@@ -233,10 +233,12 @@ siting_status siting_job_derived(siting_job *job, siting_derived *d);
  * the per-job VIX, FINDMAX, VIEWSHED and SITE-preparation kernels go to a
  * small pool of streams so they overlap, and every job's greedy SITE loop
  * runs in a single persistent work-queue launch whose slots of workgroups
- * take jobs, largest first, as they free up.  After the first synced run the
- * slots are re-planned once from the measured loop time of every job: the
- * longest jobs get a few larger slots.  The prep sequence is captured into a
- * HIP graph and replayed.  Results are bit-identical to running each job
+ * take jobs, largest first, as they free up.  The slot plan is tuned by
+ * measurement over the first synced runs (equal slots, then a mixed plan
+ * from each job's measured loop time, then the job that finished last moved
+ * onto a larger slot of its own, up to five runs), and the fastest plan
+ * measured is kept.  The prep sequence is captured into a HIP graph and
+ * replayed.  Results are bit-identical to running each job
  * alone, whatever slot runs it. */
 
 typedef struct siting_batch siting_batch;
@@ -246,9 +248,9 @@ typedef struct {
   int slot;      /* workgroups per SITE slot (0: tuned default) */
   int nslots;    /* SITE slots (0: CUs / slot) */
   int use_graph; /* 1: capture the run into a HIP graph on first use */
-  int fixed_slots; /* 1: keep equal slots; 0: after the first synced run,
-                      re-plan the slots (a few larger ones for the longest
-                      jobs) from each job's measured SITE loop time */
+  int fixed_slots; /* 1: keep equal slots; 0: tune the slot plan over the
+                      first synced runs from each job's measured SITE loop
+                      time and keep the fastest plan measured */
 } siting_batch_opts;
 
 siting_status siting_batch_create(int device, int njobs, const siting_params *params,
