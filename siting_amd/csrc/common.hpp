@@ -81,10 +81,23 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
   return v;
 }
 
+// Sum over the wavefront, in every lane (all lanes active), with DPP row
+// operations: quad swaps, half-row and row mirrors (every lane then holds
+// its row's sum), then the row broadcasts that fold rows 0-3 into lane 63.
+// A lane of a row a broadcast leaves out adds 0.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ int dpp_add_step(int v) {
+  return v + __builtin_amdgcn_update_dpp(0, v, kCtrl, kRowMask, 0xf, false);
+}
+
 __device__ __forceinline__ int wave_sum_i32(int v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-  return v;
+  v = dpp_add_step<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v = dpp_add_step<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+  v = dpp_add_step<0x141, 0xf>(v);  // row_half_mirror
+  v = dpp_add_step<0x140, 0xf>(v);  // row_mirror
+  v = dpp_add_step<0x142, 0xa>(v);  // row_bcast:15 into rows 1, 3
+  v = dpp_add_step<0x143, 0xc>(v);  // row_bcast:31 into rows 2, 3
+  return __builtin_amdgcn_readlane(v, 63);
 }
 
 // ---------------------------------------------------------------------------

@@ -296,12 +296,15 @@ __device__ __forceinline__ bool grid_sync(unsigned int *ctr, unsigned int &targe
 
 // Shifted word dk (0..nwpr) of row dr of the winner's shed, i.e. the bits of
 // terrain word w0w+dk that the winner covers (siting.cpp:408-417).
+// Both loads are unconditional (from clamped slots), so neither is waited
+// for inside a branch.
 __device__ __forceinline__ uint64_t winner_word(const uint64_t *sw, int nwpr,
                                                 int fbw, int dr, int dk) {
   const uint64_t *row = sw + (size_t)dr * nwpr;
-  uint64_t cur = dk < nwpr ? row[dk] : 0;
-  uint64_t v = cur >> fbw;
-  if (fbw != 0 && dk > 0) v |= row[dk - 1] << (64 - fbw);
+  const uint64_t cur = row[dk < nwpr ? dk : nwpr - 1];
+  const uint64_t prv = row[dk > 0 ? dk - 1 : 0];
+  uint64_t v = dk < nwpr ? cur >> fbw : 0;
+  if (fbw != 0 && dk > 0) v |= prv << (64 - fbw);
   return v;
 }
 
@@ -323,6 +326,11 @@ struct Winner {
 constexpr int kBatch = 4;
 // items per lane in flight when re-evaluating (8 spills and measured slower)
 constexpr int kEvalBatch = 4;
+// rows per lane in flight in the row-dealt re-evaluation
+#ifndef SITING_EVAL_ROWS
+#define SITING_EVAL_ROWS 4
+#endif
+constexpr int kEvalRows = SITING_EVAL_ROWS;
 
 template <int kB = kBatch>
 __device__ __forceinline__ int lane_new_bits(const uint64_t *sh, int cx, int cy,
@@ -353,10 +361,16 @@ __device__ __forceinline__ int lane_new_bits(const uint64_t *sh, int cx, int cy,
       const uint64_t *srow = sh + (size_t)row * nwpr;
       cur[u] = (ok && k < nwpr) ? srow[k] : 0;
       prv[u] = (ok && k > 0) ? srow[k - 1] : 0;
-      uint64_t c = ok ? ld_dev(&a.cum[(size_t)(xo + row) * cw + (w0c + k)]) : ~0ull;
-      const int dr = dro + row, dk = w0c + k - W.w0w;
-      if (use_w && ok && dr >= 0 && dr < W.nr && dk >= 0 && dk <= W.nwpr)
-        c |= W.word(dr, dk);
+      // unconditional: a load under a branch is waited for inside it, which
+      // would serialise the batch (a lane past the end has no shed bits, so
+      // the word it reads does not matter)
+      uint64_t c = ld_dev(ok ? &a.cum[(size_t)(xo + row) * cw + (w0c + k)] : a.cum);
+      if (use_w) {
+        const int dr = dro + row, dk = w0c + k - W.w0w;
+        const bool in = ok && dr >= 0 && dr < W.nr && dk >= 0 && dk <= W.nwpr;
+        const uint64_t wv = W.word(in ? dr : 0, in ? dk : 0);
+        c |= in ? wv : 0;
+      }
       cv[u] = c;
     }
 #pragma unroll
@@ -365,6 +379,61 @@ __device__ __forceinline__ int lane_new_bits(const uint64_t *sh, int cx, int cy,
       if (fbc != 0) moved |= prv[u] << (64 - fbc);
       s += __popcll(moved & ~cv[u]);
     }
+  }
+  return s;
+}
+
+// The same count (against cum alone) with rows dealt to lanes: lane l of
+// each wavefront of a group of gs lanes takes word slot l mod nk of row
+// l / nk + (64 / nk) * (its wavefront in the group), stepping by the
+// group's rows, so its addresses only advance by a stride; the word before
+// its slot comes from the lane before it.  For boxes whose first word slot
+// is inside the terrain and whose rows have at most 64 slots
+// (eval_rows_fit).
+__device__ __forceinline__ bool eval_rows_fit(int cy, int roi, int nwpr) {
+  return cy - roi >= 0 && nwpr + 1 <= kWave;
+}
+
+template <int kB>
+__device__ __forceinline__ int lane_new_bits_rows(const uint64_t *sh, int cx, int cy,
+                                                  const SiteArgs &a, int gl, int gs) {
+  const int roi = a.roi, nwpr = a.nwpr, cw = a.cw, n = a.n;
+  int w0c, fbc;
+  col_split(cy - roi, w0c, fbc);
+  const int nk = min((cy + roi) >> 6, cw - 1) - w0c + 1;  // word slots 0 .. nk-1
+  const int lane = gl & (kWave - 1), wi = gl >> 6;
+  const int rpw = kWave / nk;                             // rows per wavefront and step
+  const int mk = lane % nk, mr = lane / nk;
+  const bool act = mr < rpw;
+  const int rstep = (gs >> 6) * rpw;
+  const int rlo = max(0, cx - roi) - (cx - roi);
+  const int rhi = min(n - 1, cx + roi) - (cx - roi);
+  const int r0 = rlo + wi * rpw + mr;
+  const uint64_t *sp = sh + (size_t)r0 * nwpr + mk;
+  const uint64_t *cp = a.cum + (size_t)(cx - roi + r0) * cw + (w0c + mk);
+  const size_t ss = (size_t)rstep * nwpr, cs = (size_t)rstep * cw;
+  int s = 0;
+  for (int r = r0; r - r0 + rlo <= rhi; r += kB * rstep) {
+    uint64_t cur[kB], cv[kB];
+#pragma unroll
+    for (int u = 0; u < kB; u++) {
+      const bool ok = act && r + u * rstep <= rhi;
+      cur[u] = (ok && mk < nwpr) ? sp[u * ss] : 0;
+      cv[u] = ld_dev(ok ? cp + u * cs : a.cum);  // unconditional (lane_new_bits)
+    }
+#pragma unroll
+    for (int u = 0; u < kB; u++) {
+      uint64_t moved = cur[u];
+      if (fbc != 0) {
+        // every lane takes part in the shuffle; a row's first slot has none
+        const uint64_t up = (uint64_t)__shfl_up((unsigned long long)cur[u], 1, kWave);
+        const uint64_t prv = mk > 0 ? up : 0;
+        moved = (cur[u] >> fbc) | (prv << (64 - fbc));
+      }
+      s += __popcll(moved & ~cv[u]);
+    }
+    sp += kB * ss;
+    cp += kB * cs;
   }
   return s;
 }
@@ -414,8 +483,10 @@ __device__ __forceinline__ int wave_lost_bits(const uint64_t *sh, int cx, int cy
         const bool ok = it < items;
         const int q = fdiv ? (int)(((float)it + 0.5f) * inv_nk) : it / nk;
         const int x = xlo + q, j = jlo + (it - q * nk);
-        nbv[u] = ok ? ld_dev(&nbp[(size_t)(x - wx + roi) * dw + (j - w0p)]) : 0;
-        soff[u] = (x - cx + roi) * nwpr;
+        // unconditional (lane_new_bits); a lane past the end keeps nothing
+        const uint64_t v = ld_dev(ok ? &nbp[(size_t)(x - wx + roi) * dw + (j - w0p)] : nbp);
+        nbv[u] = ok ? v : 0;
+        soff[u] = ok ? (x - cx + roi) * nwpr : 0;
         kk[u] = j - w0c;
       }
 #pragma unroll
@@ -473,8 +544,8 @@ __device__ __forceinline__ int lane_new_bits_picks(const uint64_t *csh, int cx, 
       xr[u] = cx - roi + row;
       jw[u] = w0c + k;
       cv[u] = c;
-      // (no load waits on another: one memory latency per batch)
-      mv[u] = ok ? ld_dev(&a.cum[(size_t)xr[u] * cw + jw[u]]) : 0;
+      // unconditional loads, so that none waits on another (lane_new_bits)
+      mv[u] = ld_dev(ok ? &a.cum[(size_t)xr[u] * cw + jw[u]] : a.cum);
     }
     for (int j = 0; j < np; j++) {
       const uint32_t pq = ppos[j];
@@ -482,11 +553,24 @@ __device__ __forceinline__ int lane_new_bits_picks(const uint64_t *csh, int cx, 
       const uint64_t *psh = a.sheds + (size_t)pid[j] * S;
       int w0p, fbp;
       col_split(py - roi, w0p, fbp);
+      uint64_t pc[kBatch], pv[kBatch];
+      bool in[kBatch];
 #pragma unroll
       for (int u = 0; u < kBatch; u++) {
         const int dr = xr[u] - px + roi, dk = jw[u] - w0p;
-        if (dr >= 0 && dr < nr && dk >= 0 && dk <= nwpr)
-          mv[u] |= winner_word(psh, nwpr, fbp, dr, dk);
+        in[u] = dr >= 0 && dr < nr && dk >= 0 && dk <= nwpr;
+        const uint64_t *prow = psh + (size_t)(in[u] ? dr : 0) * nwpr;
+        const int kc = in[u] ? dk : 0;
+        pc[u] = prow[kc < nwpr ? kc : nwpr - 1];
+        pv[u] = prow[kc > 0 ? kc - 1 : 0];
+        pc[u] = kc < nwpr ? pc[u] : 0;
+        pv[u] = kc > 0 ? pv[u] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; u++) {
+        uint64_t pw = pc[u] >> fbp;
+        if (fbp != 0) pw |= pv[u] << (64 - fbp);
+        mv[u] |= in[u] ? pw : 0;
       }
     }
 #pragma unroll
@@ -695,9 +779,19 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   auto unit_top2 = [&](int ul) {
     const int lo = OFF(ul, 0), hi = OFF(ul, kpu);
     Top2 t{0, 0, 0, 0};
-    for (int i = lo + lane; i < hi; i += kWave) {
-      const uint8_t u = U[i];
-      if (!(u & 1)) top2_push(t, cand_key(T[i], C[i]), P[i] | ((u & 2) ? 0x80000000u : 0u));
+    // two candidates per lane in flight, every load unconditional (a load
+    // under a branch is waited for inside it); a lane past the end re-reads
+    // the unit's first candidate and pushes nothing (key 0)
+    for (int i0 = lo + lane; i0 < hi; i0 += 2 * kWave) {
+      const int i1 = i0 + kWave;
+      const int j1 = i1 < hi ? i1 : lo;
+      const uint8_t u0 = U[i0], u1 = U[j1];
+      const int t0 = T[i0], t1 = T[j1];
+      const int c0 = C[i0], c1 = C[j1];
+      const uint32_t p0 = P[i0], p1 = P[j1];
+      top2_push(t, (u0 & 1) ? 0 : cand_key(t0, c0), p0 | ((u0 & 2) ? 0x80000000u : 0u));
+      top2_push(t, (i1 >= hi || (u1 & 1)) ? 0 : cand_key(t1, c1),
+                p1 | ((u1 & 2) ? 0x80000000u : 0u));
     }
     t = wave_top2(t);
     if (lane == 0) {
@@ -851,10 +945,15 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         for (int u0 = wid * kWave + lane; u0 < nunits; u0 += 2 * kPickWaves * kWave) {
           const int u1 = u0 + kPickWaves * kWave;
           const bool has1 = u1 < nunits;
+          // every load unconditional (a load under a branch is waited for
+          // inside it); a missing second unit reads the first's and drops it
+          const int u1s = has1 ? u1 : u0;
           const uint64_t a0 = ld_dev(&kk[2 * u0]), a1 = ld_dev(&kk[2 * u0 + 1]);
           const uint32_t q0 = ld_dev(&kp[2 * u0]), q1 = ld_dev(&kp[2 * u0 + 1]);
-          const uint64_t b0 = has1 ? ld_dev(&kk[2 * u1]) : 0, b1 = has1 ? ld_dev(&kk[2 * u1 + 1]) : 0;
-          const uint32_t r0 = has1 ? ld_dev(&kp[2 * u1]) : 0, r1 = has1 ? ld_dev(&kp[2 * u1 + 1]) : 0;
+          const uint64_t b0v = ld_dev(&kk[2 * u1s]), b1v = ld_dev(&kk[2 * u1s + 1]);
+          const uint32_t r0v = ld_dev(&kp[2 * u1s]), r1v = ld_dev(&kp[2 * u1s + 1]);
+          const uint64_t b0 = has1 ? b0v : 0, b1 = has1 ? b1v : 0;
+          const uint32_t r0 = has1 ? r0v : 0, r1 = has1 ? r1v : 0;
           top3_push(h, a0, q0);
           top3_push(h, a1, q1 | kPickLast);
           top3_push(h, b0, r0);
@@ -1397,8 +1496,13 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         }
         const int i = rng_lo[lo] + item - (lo > 0 ? rng_end[lo - 1] : 0);
         const unsigned int bit = 1u << (i & 31);
-        if (!(U[i] & 1) && !(atomicOr(&seen[i >> 5], bit) & bit)) {
-          const uint32_t ps = P[i];
+        // the candidate's flags, position and area in one round trip (the
+        // empty asm keeps the loads together, before any use)
+        const uint8_t ui = U[i];
+        const uint32_t ps = P[i];
+        const int ti = T[i];
+        asm volatile("" ::"v"((int)ui), "v"(ps), "v"(ti));
+        if (!(ui & 1) && !(atomicOr(&seen[i >> 5], bit) & bit)) {
           const int cx = (int)(ps >> 16), cy = (int)(ps & 0xffffu);
           int m = 0;
           for (int j = 0; j < kc; j++) {
@@ -1406,7 +1510,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
             const int ey = cy - (int)(bw_pos[j] & 0xffffu);
             m += ex * ex + ey * ey <= r2;
           }
-          if (m && hybrid && ((U[i] & 2) || T[i] < tau)) {
+          if (m && hybrid && ((ui & 2) || ti < tau)) {
             my_u += m;  // keep the old area as a bound
             U[i] |= 2;
             dirty[((cx / a.cs) * Q + (cy / a.cs) % Q) / G] = 1;
@@ -1507,14 +1611,28 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         while (2 * gw * lcnt <= kLoopWaves) gw *= 2;
         const int ngroups = kLoopWaves / gw;
         const int grp = wid / gw, gl = (wid - grp * gw) * kWave + lane;
-        for (int e = grp; e < lcnt; e += ngroups) {
-          const int i = list_i[e];
-          const uint32_t ps = P[i];
-          const int cx = (int)(ps >> 16), cy = (int)(ps & 0xffffu);
-          const uint64_t *sh = a.sheds + (size_t)C[i] * S;
-          const int s =
-              wave_sum_i32(lane_new_bits<kEvalBatch>(sh, cx, cy, a, W0, gl, gw * kWave, false));
-          if (lane == 0) atomicAdd(&list_acc[e], s);
+        // the wavefront's candidates' positions and ids, 64 at a time, in one
+        // round trip; then one candidate after another
+        for (int e0 = grp; e0 < lcnt; e0 += kWave * ngroups) {
+          const int em = e0 + lane * ngroups;
+          uint32_t mps = 0, mid = 0;
+          if (em < lcnt) {
+            const int i = list_i[em];
+            mps = P[i];
+            mid = (uint32_t)C[i];
+          }
+          const int ncand = min(kWave, (lcnt - e0 + ngroups - 1) / ngroups);
+          for (int j = 0; j < ncand; j++) {
+            const uint32_t ps = (uint32_t)__builtin_amdgcn_readlane((int)mps, j);
+            const uint32_t cid = (uint32_t)__builtin_amdgcn_readlane((int)mid, j);
+            const int cx = (int)(ps >> 16), cy = (int)(ps & 0xffffu);
+            const uint64_t *sh = a.sheds + (size_t)cid * S;
+            const int s =
+                wave_sum_i32(eval_rows_fit(cy, roi, nwpr)
+                                 ? lane_new_bits_rows<kEvalRows>(sh, cx, cy, a, gl, gw * kWave)
+                                 : lane_new_bits<kEvalBatch>(sh, cx, cy, a, W0, gl, gw * kWave, false));
+            if (lane == 0) atomicAdd(&list_acc[e0 + j * ngroups], s);
+          }
         }
         __syncthreads();
         for (int e = tid; e < lcnt; e += kLoopThreads) {
