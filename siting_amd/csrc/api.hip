@@ -213,7 +213,7 @@ Layout layout_for(const siting_params &p, const siting_derived &d) {
   put(L.sheds, nw * (size_t)d.shed_words * 8);
   put(L.area, nw * 4);
   put(L.used, nw);
-  put(L.cum, (size_t)p.nrows * d.cumnwpr * 8);
+  put(L.cum, cum_words(p.nrows, d.cumnwpr) * 8);
   put(L.rowmax, 2 * (size_t)site_key_rows(p.nrows, p.roi) * 8);
   put(L.order, nw * 4);
   put(L.state, sizeof(SiteState));
@@ -726,7 +726,7 @@ siting_status siting_site_it(int nrows, int roi, int intervis, int nsheds,
   CHECK(dalloc(dsh, sbytes));
   CHECK(dalloc(darea, nw * 4));
   CHECK(dalloc(dused, nw));
-  CHECK(dalloc(dcum, (size_t)nrows * g.d.cumnwpr * 8));
+  CHECK(dalloc(dcum, cum_words(nrows, g.d.cumnwpr) * 8));
   CHECK(dalloc(drm, 2 * (size_t)site_key_rows(nrows, roi) * 8));
   CHECK(dalloc(dord, nw * 4));
   CHECK(dalloc(dst, sizeof(SiteState)));
@@ -761,11 +761,11 @@ siting_status siting_site_it(int nrows, int roi, int intervis, int nsheds,
 // ---------------------------------------------------------------------------
 namespace siting {
 
-// A plan the batch has run, with each job's loop end and the SITE launch
+// A plan the batch has run, with each job's loop time and the SITE launch
 // time it measured.
 struct TriedPlan {
   SlotClasses classes;
-  std::vector<double> end;
+  std::vector<double> dur;
   double span_ms;
 };
 
@@ -906,11 +906,13 @@ void batch_drop_graph(siting_batch *b) {
 // beside it:
 //  - run 1 uses equal slots of the base size;
 //  - run 2 the mixed-slot plan of the square-root model below;
-//  - each later run, up to kTuneRuns, moves the job that finished last in
-//    the fastest plan measured so far onto a slot of its own twice as large
-//    (at least 32 groups), paid for with base slots; a move that does not
-//    shorten the launch is undone, and the next tries the job that finished
-//    last but one, and so on;
+//  - each later run, up to kTuneRuns, moves the job whose loop ran longest
+//    in the fastest plan measured so far onto a slot of its own twice as
+//    large (at least 32 groups), paid for with base slots; a move that does
+//    not shorten the launch is undone, and the next tries the job that ran
+//    longest but one, and so on.  (Moving the job that finished last was
+//    worse: that is usually a small job that waited in a queue, and its slot
+//    took base slots from the queue's other small jobs, 77 -> 437 ms);
 //  - then the fastest plan measured is kept for good.
 // (Alone, a 3365^2 roi-150 job of the uneven batch takes 84 ms on 8 groups,
 // 26 on 32 and 18 on 64; a 2593^2 roi-150 one with 2761 rounds 230, 90 and
@@ -924,18 +926,15 @@ siting_status batch_tune(siting_batch *b) {
   std::vector<unsigned long long> t((size_t)2 * njobs);
   SITING_HIP(hipMemcpy(t.data(), b->jobt, sizeof(unsigned long long) * t.size(),
                        hipMemcpyDeviceToHost));
-  unsigned long long first = ~0ull;
-  for (int i = 0; i < njobs; i++) first = std::min(first, t[(size_t)2 * i]);
-  std::vector<double> end((size_t)njobs), ms((size_t)njobs), g((size_t)njobs);
+  std::vector<double> ms((size_t)njobs), g((size_t)njobs);
   for (int i = 0; i < njobs; i++) {
     const unsigned long long a = t[(size_t)2 * i], e = t[(size_t)2 * i + 1];
-    end[(size_t)i] = e > first ? (double)(e - first) * 1e-5 : 0.0;  // 100 MHz clock
-    ms[(size_t)i] = e > a ? (double)(e - a) * 1e-5 : 0.0;
+    ms[(size_t)i] = e > a ? (double)(e - a) * 1e-5 : 0.0;  // 100 MHz clock
     g[(size_t)i] = site_loop_groups(b->jobs[(size_t)i]->g, b->cus, b->cap[(size_t)i]);
   }
   float span = 0;
   SITING_HIP(hipEventElapsedTime(&span, b->ts, b->t1));
-  b->tried.push_back({b->cur_classes, end, (double)span});
+  b->tried.push_back({b->cur_classes, ms, (double)span});
   b->tune_left--;
   size_t best = 0;
   for (size_t k = 1; k < b->tried.size(); k++)
@@ -950,13 +949,13 @@ siting_status batch_tune(siting_batch *b) {
                        },
                        b->budget, b->slot, &next);
     }
-    // local moves from the best plan: its jobs by end time, latest first,
+    // local moves from the best plan: its jobs by loop time, longest first,
     // skipping the moves already tried from it
     const auto &bp = b->tried[best];
     std::vector<int> late((size_t)njobs);
     for (int i = 0; i < njobs; i++) late[(size_t)i] = i;
     std::stable_sort(late.begin(), late.end(),
-                     [&](int x, int y) { return bp.end[(size_t)x] > bp.end[(size_t)y]; });
+                     [&](int x, int y) { return bp.dur[(size_t)x] > bp.dur[(size_t)y]; });
     for (size_t r = 0; !have && r < late.size() && r < 8; r++) {
       SlotClasses c = bp.classes;
       if (!move_to_own_slot(b->cus / 2, &c, late[r])) continue;

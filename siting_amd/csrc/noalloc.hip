@@ -55,7 +55,7 @@ NoAllocLayout noalloc_layout(int nrows, int roi, int nsheds, int intervis, bool 
   };
   const size_t cw = ((size_t)nrows + 63) / 64;
   put(L.area, 4 * (size_t)nsheds);
-  put(L.cum, 8 * (size_t)nrows * cw);
+  put(L.cum, 8 * cum_words(nrows, (int)cw));
   put(L.rowmax, 2 * (size_t)site_key_rows(nrows, roi) * 8);
   put(L.order, own_order ? 4 * (size_t)nsheds : 0);
   put(L.state, sizeof(SiteState));

@@ -204,8 +204,8 @@ __global__ void site_bucket_scatter(SiteArgs a) {
 __global__ __launch_bounds__(256) void site_first_pass(SiteArgs a) {
   const int lane = threadIdx.x & 63;
   const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const size_t cum_words = (size_t)a.n * a.cw;
-  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < cum_words;
+  const size_t ncum = cum_words(a.n, a.cw);
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < ncum;
        i += (size_t)gridDim.x * 256)
     a.cum[i] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -364,7 +364,7 @@ __device__ __forceinline__ int lane_new_bits(const uint64_t *sh, int cx, int cy,
       // unconditional: a load under a branch is waited for inside it, which
       // would serialise the batch (a lane past the end has no shed bits, so
       // the word it reads does not matter)
-      uint64_t c = ld_dev(ok ? &a.cum[(size_t)(xo + row) * cw + (w0c + k)] : a.cum);
+      uint64_t c = ld_dev(ok ? &a.cum[cum_index(xo + row, w0c + k, cw)] : a.cum);
       if (use_w) {
         const int dr = dro + row, dk = w0c + k - W.w0w;
         const bool in = ok && dr >= 0 && dr < W.nr && dk >= 0 && dk <= W.nwpr;
@@ -410,8 +410,8 @@ __device__ __forceinline__ int lane_new_bits_rows(const uint64_t *sh, int cx, in
   const int rhi = min(n - 1, cx + roi) - (cx - roi);
   const int r0 = rlo + wi * rpw + mr;
   const uint64_t *sp = sh + (size_t)r0 * nwpr + mk;
-  const uint64_t *cp = a.cum + (size_t)(cx - roi + r0) * cw + (w0c + mk);
-  const size_t ss = (size_t)rstep * nwpr, cs = (size_t)rstep * cw;
+  const size_t ss = (size_t)rstep * nwpr;
+  const int xo = cx - roi, jc = w0c + mk;
   int s = 0;
   for (int r = r0; r - r0 + rlo <= rhi; r += kB * rstep) {
     uint64_t cur[kB], cv[kB];
@@ -419,7 +419,8 @@ __device__ __forceinline__ int lane_new_bits_rows(const uint64_t *sh, int cx, in
     for (int u = 0; u < kB; u++) {
       const bool ok = act && r + u * rstep <= rhi;
       cur[u] = (ok && mk < nwpr) ? sp[u * ss] : 0;
-      cv[u] = ld_dev(ok ? cp + u * cs : a.cum);  // unconditional (lane_new_bits)
+      // unconditional (lane_new_bits)
+      cv[u] = ld_dev(ok ? &a.cum[cum_index(xo + r + u * rstep, jc, cw)] : a.cum);
     }
 #pragma unroll
     for (int u = 0; u < kB; u++) {
@@ -433,7 +434,6 @@ __device__ __forceinline__ int lane_new_bits_rows(const uint64_t *sh, int cx, in
       s += __popcll(moved & ~cv[u]);
     }
     sp += kB * ss;
-    cp += kB * cs;
   }
   return s;
 }
@@ -545,7 +545,7 @@ __device__ __forceinline__ int lane_new_bits_picks(const uint64_t *csh, int cx, 
       jw[u] = w0c + k;
       cv[u] = c;
       // unconditional loads, so that none waits on another (lane_new_bits)
-      mv[u] = ld_dev(ok ? &a.cum[(size_t)xr[u] * cw + jw[u]] : a.cum);
+      mv[u] = ld_dev(ok ? &a.cum[cum_index(xr[u], jw[u], cw)] : a.cum);
     }
     for (int j = 0; j < np; j++) {
       const uint32_t pq = ppos[j];
@@ -1391,7 +1391,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         const int x = W.wx - roi + dr, j = W.w0w + dk;
         if (x < 0 || x >= n || j < 0 || j >= cw) continue;
         const uint64_t v = W.word(dr, dk);
-        if (v) atomicOr((unsigned long long *)&a.cum[(size_t)x * cw + j], (unsigned long long)v);
+        if (v) atomicOr((unsigned long long *)&a.cum[cum_index(x, j, cw)], (unsigned long long)v);
       }
     } else if (k > 0) {
       // every pick: OR its shifted shed into cum and keep the bits that were
@@ -1415,7 +1415,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           const uint64_t *sw = a.sheds + (size_t)(int)(~(uint32_t)bw_key[i]) * S;
           const uint64_t v = winner_word(sw, nwpr, fbw, dr, dk);
           if (v) {
-            const uint64_t old = atomicOr((unsigned long long *)&a.cum[(size_t)x * cw + j],
+            const uint64_t old = atomicOr((unsigned long long *)&a.cum[cum_index(x, j, cw)],
                                           (unsigned long long)v);
             nbv = v & ~old;
           }
@@ -1517,7 +1517,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           } else if (m) {
             int acc = m;
             if (!stop && a.intervis) {  // cpp:452-457
-              uint64_t word = ld_dev(&a.cum[(size_t)cx * cw + (cy >> 6)]);
+              uint64_t word = ld_dev(&a.cum[cum_index(cx, cy >> 6, cw)]);
               if (!ivk) {  // one pick: cum | its bits
                 const int dr = cx - W.wx + roi, dk = (cy >> 6) - W.w0w;
                 if (dr >= 0 && dr < nr && dk >= 0 && dk <= nwpr) word |= W.word(dr, dk);
