@@ -1965,10 +1965,11 @@ siting_status site_make_args(const Geometry &g, const int32_t *obs,
     const char *e = std::getenv("SITING_SITE_IV_MULTI");
     a.iv_multi = g.p.intervis && !(e && *e && std::atoi(e) == 0);
     // rule: 8 for roi up to 40 (measured: a 7594^2 roi-25 request's SITE
-    // 751 -> 651 ms), none beyond (a 7857^2 roi-50 one: 229 ms with none,
-    // 254-398 with 1-8); a job's tuning replaces the rule,
+    // 816 -> 659 ms) and above 75 (the tuner's roi-100 to 200 classes:
+    // 5-12 % faster), none between (a 7857^2 roi-50 request: 231 ms with
+    // none, 282-389 with 2-16); a job's tuning replaces the rule,
     // SITING_SITE_WALK_EVALS overrides both
-    int ev = roi <= 40 ? 8 : 0;
+    int ev = (roi <= 40 || roi > 75) ? 8 : 0;
     if (g.t.site_walk_evals >= 1 && g.t.site_walk_evals <= kMaxBatch + 1)
       ev = g.t.site_walk_evals - 1;
     const char *w = std::getenv("SITING_SITE_WALK_EVALS");

@@ -92,7 +92,7 @@ def rule_choice(roi: int, ntests: int, n: int, nsheds: int) -> dict:
     return {"vix_halo": 32 if roi >= 64 and ntests >= 64 else 16,
             "viewshed_threads": 256, "viewshed_mode": modes[0],
             "site_q": 3 if q_list >= 3000.0 else 2, "site_tau_shift": 5,
-            "site_walk_evals": 9 if roi <= 40 else 1}
+            "site_walk_evals": 9 if roi <= 40 or roi > 75 else 1}
 
 
 def tune_class(key: str, roi: int, ntests: int, edge: int, reps: int, log):
