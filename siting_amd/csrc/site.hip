@@ -510,7 +510,10 @@ __device__ __forceinline__ int wave_lost_bits(const uint64_t *sh, int cx, int cy
 // picks (positions ppos[], shed indices pid[], in LDS), i.e. its area once
 // they are taken (intervis 0) or, for one pick, its area bound after that
 // pick (intervis 1): lane gl of gs lanes' share.  Items are (shed row,
-// terrain word) pairs as in lane_new_bits.
+// terrain word) pairs as in lane_new_bits, kIt per lane in flight; the
+// picks' words are read kPG picks at a time, and every load of a group is
+// issued before any is used (one round trip per group).
+template <int kIt, int kPG>
 __device__ __forceinline__ int lane_new_bits_picks(const uint64_t *csh, int cx, int cy, int np,
                                                    const uint32_t *ppos, const uint32_t *pid,
                                                    const SiteArgs &a, int gl, int gs) {
@@ -527,54 +530,61 @@ __device__ __forceinline__ int lane_new_bits_picks(const uint64_t *csh, int cx, 
   const float inv_nk = 1.0f / (float)nk;
   const bool fdiv = items < (1 << 21);
   int s = 0;
-  for (int base = 0; base < items; base += gs * kBatch) {
-    uint64_t cv[kBatch], mv[kBatch];
-    int xr[kBatch], jw[kBatch];
+  for (int base = 0; base < items; base += gs * kIt) {
+    uint64_t cur[kIt], prv[kIt], mv[kIt];
+    int xr[kIt], jw[kIt];
 #pragma unroll
-    for (int u = 0; u < kBatch; u++) {
+    for (int u = 0; u < kIt; u++) {
       const int it = base + u * gs + gl;
       const bool ok = it < items;
       const int q = fdiv ? (int)(((float)it + 0.5f) * inv_nk) : it / nk;
       const int row = rlo + q, k = klo + (it - q * nk);
       const uint64_t *srow = csh + (size_t)row * nwpr;
-      const uint64_t cur = (ok && k < nwpr) ? srow[k] : 0;
-      const uint64_t prv = (ok && k > 0) ? srow[k - 1] : 0;
-      uint64_t c = cur >> fbc;
-      if (fbc != 0) c |= prv << (64 - fbc);
+      cur[u] = (ok && k < nwpr) ? srow[k] : 0;
+      prv[u] = (ok && k > 0) ? srow[k - 1] : 0;
       xr[u] = cx - roi + row;
       jw[u] = w0c + k;
-      cv[u] = c;
       // unconditional loads, so that none waits on another (lane_new_bits)
       mv[u] = ld_dev(ok ? &a.cum[cum_index(xr[u], jw[u], cw)] : a.cum);
     }
-    for (int j = 0; j < np; j++) {
-      const uint32_t pq = ppos[j];
-      const int px = (int)(pq >> 16), py = (int)(pq & 0xffffu);
-      const uint64_t *psh = a.sheds + (size_t)pid[j] * S;
-      int w0p, fbp;
-      col_split(py - roi, w0p, fbp);
-      uint64_t pc[kBatch], pv[kBatch];
-      bool in[kBatch];
+    for (int j0 = 0; j0 < np; j0 += kPG) {
+      uint64_t pc[kIt][kPG], pv[kIt][kPG];
+      int w0p[kPG], fbp[kPG], pxs[kPG];
 #pragma unroll
-      for (int u = 0; u < kBatch; u++) {
-        const int dr = xr[u] - px + roi, dk = jw[u] - w0p;
-        in[u] = dr >= 0 && dr < nr && dk >= 0 && dk <= nwpr;
-        const uint64_t *prow = psh + (size_t)(in[u] ? dr : 0) * nwpr;
-        const int kc = in[u] ? dk : 0;
-        pc[u] = prow[kc < nwpr ? kc : nwpr - 1];
-        pv[u] = prow[kc > 0 ? kc - 1 : 0];
-        pc[u] = kc < nwpr ? pc[u] : 0;
-        pv[u] = kc > 0 ? pv[u] : 0;
+      for (int t = 0; t < kPG; t++) {
+        const int j = j0 + t < np ? j0 + t : np - 1;  // a missing pick repeats one
+        const uint32_t pq = ppos[j];
+        pxs[t] = (int)(pq >> 16);
+        col_split((int)(pq & 0xffffu) - roi, w0p[t], fbp[t]);
+        const uint64_t *psh = a.sheds + (size_t)pid[j] * S;
+#pragma unroll
+        for (int u = 0; u < kIt; u++) {
+          const int dr = xr[u] - pxs[t] + roi, dk = jw[u] - w0p[t];
+          const bool in = dr >= 0 && dr < nr && dk >= 0 && dk <= nwpr;
+          const uint64_t *prow = psh + (size_t)(in ? dr : 0) * nwpr;
+          const int kc = in ? dk : 0;
+          pc[u][t] = prow[kc < nwpr ? kc : nwpr - 1];
+          pv[u][t] = prow[kc > 0 ? kc - 1 : 0];
+        }
       }
 #pragma unroll
-      for (int u = 0; u < kBatch; u++) {
-        uint64_t pw = pc[u] >> fbp;
-        if (fbp != 0) pw |= pv[u] << (64 - fbp);
-        mv[u] |= in[u] ? pw : 0;
+      for (int t = 0; t < kPG; t++) {
+#pragma unroll
+        for (int u = 0; u < kIt; u++) {
+          const int dr = xr[u] - pxs[t] + roi, dk = jw[u] - w0p[t];
+          const bool in = j0 + t < np && dr >= 0 && dr < nr && dk >= 0 && dk <= nwpr;
+          uint64_t pw = dk < nwpr ? pc[u][t] >> fbp[t] : 0;
+          if (fbp[t] != 0 && dk > 0) pw |= pv[u][t] << (64 - fbp[t]);
+          mv[u] |= in ? pw : 0;
+        }
       }
     }
 #pragma unroll
-    for (int u = 0; u < kBatch; u++) s += __popcll(cv[u] & ~mv[u]);
+    for (int u = 0; u < kIt; u++) {
+      uint64_t c = cur[u] >> fbc;
+      if (fbc != 0) c |= prv[u] << (64 - fbc);
+      s += __popcll(c & ~mv[u]);
+    }
   }
   return s;
 }
@@ -1003,7 +1013,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
             __builtin_amdgcn_wave_barrier();
-            return wave_sum_i32(lane_new_bits_picks(a.sheds + (size_t)cid * S, (int)(cpos >> 16),
+            return wave_sum_i32(lane_new_bits_picks<kBatch, 1>(a.sheds + (size_t)cid * S, (int)(cpos >> 16),
                                                     (int)(cpos & 0xffffu), np, mb_ppos, mb_pid, a,
                                                     lane, kWave));
           }
@@ -1023,7 +1033,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           mb_n++;
           if (lane == 0) __hip_atomic_store(&mb_seq, mb_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-          const int own = wave_sum_i32(lane_new_bits_picks(
+          const int own = wave_sum_i32(lane_new_bits_picks<1, 4>(
               a.sheds + (size_t)cid * S, (int)(cpos >> 16), (int)(cpos & 0xffffu),
               __popcll(picks), mb_ppos, mb_pid, a, lane, kLoopThreads));
           unsigned long long acc = 0;
@@ -1248,7 +1258,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           if (q < 0) break;
           seen = q;
           const uint32_t cpos = mb_cpos;
-          const int part = wave_sum_i32(lane_new_bits_picks(
+          const int part = wave_sum_i32(lane_new_bits_picks<1, 4>(
               a.sheds + (size_t)mb_cid * S, (int)(cpos >> 16), (int)(cpos & 0xffffu), mb_np,
               mb_ppos, mb_pid, a, wid * kWave + lane, kLoopThreads));
           if (lane == 0)
