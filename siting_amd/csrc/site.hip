@@ -29,17 +29,17 @@
 //                       walks, identically, down the global key order (at
 //                       most 32 picks).  A key more than 2*roi from every
 //                       earlier pick of the round is exact and is taken.
-//                       Intervis 0: a nearer key is re-keyed to its exact
-//                       area after those picks (computed by the whole
-//                       workgroup) and goes back into the order, as the
-//                       sequential loop recomputed it after each pick.
-//                       Intervis 1: a masked candidate may gain area when a
-//                       pick uncovers its cell, so a key is taken only if it
-//                       beats every such candidate's bound (per-cell records
-//                       of the masked bounds, the largest computed exactly
-//                       on demand); a nearer key ends the round.  A walk
-//                       that read cum is followed by a grid barrier before
-//                       step 3 writes it.
+//                       A nearer key is re-keyed to its exact area after
+//                       those picks (computed by the whole workgroup; under
+//                       intervis, 0 while its own cell is uncovered) and
+//                       goes back into the order, as the sequential loop
+//                       recomputed it after each pick.  Intervis 1: a masked
+//                       candidate may gain area when a pick uncovers its
+//                       cell, so a key is taken only if it beats every such
+//                       candidate's bound (per-cell records of the masked
+//                       bounds, the largest computed exactly on demand).  A
+//                       walk that read cum is followed by a grid barrier
+//                       before step 3 writes it.
 //                    2. counters advance pick by pick; the 95% rule may end
 //                       the round after any pick.
 //                    3. the picks' sheds, shifted to terrain words, are ORed
@@ -918,7 +918,8 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   // the walk's helpers (see mb_seq): rounds that may compute areas
   // (a shed box of at most kWave * kBatch items is computed by the walking
   // wavefront alone: the handoff would cost more than it saves)
-  const bool walk_help = (a.intervis ? (ivm && a.walk_evals > 0) : a.near_evals > 0) &&
+  const bool walk_help =
+      (a.intervis ? ivm && (a.walk_evals > 0 || a.near_evals > 0) : a.near_evals > 0) &&
                          (2 * roi + 1) * (nwpr + 1) > kWave * kBatch;
   while (started) {
     if (tid == 0) mb_seq = 0;
@@ -936,9 +937,9 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     // follow the descending order of all published unit keys, as the
     // sequential loop would pick them: a key far (> 2*roi) from every
     // earlier pick of the round is exact; a nearer one is re-keyed to its
-    // exact area after those picks (intervis 0; areas only shrink, so it
-    // goes back into the order) or ends the round (intervis 1, unless the
-    // cell bounds admit it).  A unit's second key sorts below its first, so
+    // exact area after those picks (a positive key's area only shrinks, so
+    // it goes back into the order); under intervis a key is taken only if
+    // the cell bounds of the masked candidates admit it.  A unit's second key sorts below its first, so
     // the order needs no per-unit state; taking a unit's second key ends
     // the round, since its third is unknown.
     //
@@ -1116,15 +1117,38 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           }
           return false;
         };
-        // intervis 0: a key within 2*roi of an earlier pick of the round is
-        // re-keyed -- its exact area once the earlier picks within 2*roi
-        // are taken, as the sequential loop recomputed it after each of
-        // them -- and joins the rekeyed set (r_*, exact through the first
-        // r_mark picks) instead of ending the round; the largest of the
-        // lanes' keys and the rekeyed set goes next (areas only shrink, so a
-        // lane's key bounds the area of its candidate).  At most
+        // A key within 2*roi of an earlier pick of the round is re-keyed --
+        // its exact area once the earlier picks within 2*roi are taken, as
+        // the sequential loop recomputed it after each of them -- and joins
+        // the rekeyed set (r_*, exact through the first r_mark picks)
+        // instead of ending the round; the largest of the lanes' keys and the
+        // rekeyed set goes next (a positive key's area only shrinks, so a
+        // lane's key bounds the area of its candidate; masked candidates,
+        // which may gain, are covered by the cell bounds).  At most
         // a.near_evals such computations per round.
         int rn = 0, nevals = 0, npair = 0;
+        // A re-keyed candidate's area once the earlier picks within 2*roi of
+        // it are taken.  Intervis: 0 unless its own cell is covered by cum or
+        // one of those picks' sheds (siting.cpp:452-457; only picks within
+        // roi can cover it); its area can then only have shrunk, since a
+        // candidate whose key is positive was not masked.
+        auto rekey_area = [&](uint32_t cid, uint32_t cpos, uint64_t nearp) -> int {
+          if (a.intervis) {
+            const int ccx = (int)(cpos >> 16), ccy = (int)(cpos & 0xffffu);
+            bool cov = false;
+            if ((nearp >> lane) & 1) {
+              const int dr = ccx - (int)(pp >> 16) + roi, dc = ccy - (int)(pp & 0xffffu) + roi;
+              if (dr >= 0 && dr < nr && dc >= 0 && dc < nr) {
+                const uint64_t *psh = a.sheds + (size_t)(~(uint32_t)pk) * S;
+                cov = (psh[(size_t)dr * nwpr + (dc >> 6)] >> (63 - (dc & 63))) & 1u;
+              }
+            }
+            const uint64_t cwd = ld_dev(&a.cum[cum_index(ccx, ccy >> 6, cw)]);
+            const bool vis = ((cwd >> (63 - (ccy & 63))) & 1u) || __ballot(cov) != 0;
+            if (!vis) return 0;
+          }
+          return walk_area(cid, cpos, nearp);
+        };
         while (k < kmax) {
           const uint64_t bv = wave_max_u64_dpp(h.k[0]);
           if (rn > 0) {
@@ -1146,7 +1170,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
               if (__ballot(fresh)) {  // picks since its last computation reach it
                 if (nevals >= a.near_evals) break;
                 nevals++;
-                const int area = walk_area(~(uint32_t)rv, rpos, nearp);
+                const int area = rekey_area(~(uint32_t)rv, rpos, nearp);
                 if (lane == 0) {
                   r_key[e] = ((uint64_t)(uint32_t)area << 32) | (uint32_t)rv;
                   r_mark[e] = k;
@@ -1155,6 +1179,10 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
                 continue;
               }
               if ((rv >> 32) == 0) break;  // no candidate adds area
+              if (ivm) {  // it must beat every masked candidate's bound, as below
+                for (; nwin < k; nwin++) window_scan(nwin);
+                if ((uint32_t)(rv >> 32) <= bound || listed_block((int)(rv >> 32))) break;
+              }
               // take it; it was in the update list of each earlier pick
               // within 2*roi (cpp:540-544), which leaves every pick out
               if (lane == k) {
@@ -1187,11 +1215,9 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           }
           const uint64_t nearp = __ballot(near);
           if (nearp) {
-            // intervis: a near candidate may gain area (the mask lifts), so
-            // the round ends
-            if (a.intervis || rn == kMaxRekey || nevals >= a.near_evals) break;
+            if (rn == kMaxRekey || nevals >= a.near_evals) break;
             nevals++;
-            const int area = walk_area(~(uint32_t)bv, pos & ~kPickLast, nearp);
+            const int area = rekey_area(~(uint32_t)bv, pos & ~kPickLast, nearp);
             if (lane == 0) {
               r_key[rn] = ((uint64_t)(uint32_t)area << 32) | (uint32_t)bv;
               r_pos[rn] = pos & ~kPickLast;
@@ -1985,13 +2011,16 @@ siting_status site_make_args(const Geometry &g, const int32_t *obs,
     const char *w = std::getenv("SITING_SITE_WALK_EVALS");
     if (w && *w) ev = std::max(0, std::min(kMaxBatch, std::atoi(w)));
     a.walk_evals = ev;
-    // intervis 0: 8 rekeys per round (measured on roi 25-150 jobs of 894^2 to
-    // 4096^2: 10-33 % less SITE time, 8 and 16 alike, 2 to 4 behind);
-    // SITING_SITE_NEAR_EVALS overrides
-    int ne = 8;
+    // rekeys per round: 8 (intervis 0, measured on roi 25-150 jobs of 894^2
+    // to 4096^2: 10-33 % less SITE time, 8 and 16 alike, 2 to 4 behind);
+    // under intervis only where the walk's exact computations pay, by the
+    // same choice (the 2048^2 roi-80 job: 2867 -> 1560 rounds, SITE 137 ->
+    // 113 ms; a 7857^2 roi-50 request, where they are off: 2 % slower with
+    // them).  SITING_SITE_NEAR_EVALS overrides.
+    int ne = g.p.intervis ? (a.iv_multi && ev > 0 ? 8 : 0) : 8;
     const char *ne_env = std::getenv("SITING_SITE_NEAR_EVALS");
     if (ne_env && *ne_env) ne = std::max(0, std::min(kMaxBatch, std::atoi(ne_env)));
-    a.near_evals = g.p.intervis ? 0 : ne;
+    a.near_evals = g.p.intervis && !a.iv_multi ? 0 : ne;
   }
   a.n = n;
   a.roi = roi;
