@@ -245,7 +245,7 @@ struct SiteArgs {
   unsigned int *sseen;  // [G][nsheds/32 + 1] seen bitmaps of global-mode groups
   uint64_t *phash;   // [2][kMaxGroups] per-round hash of each workgroup's picks
   int32_t *sB;       // intervis: [nsheds] area bounds of global-mode workgroups
-  uint64_t *cbnd;    // intervis: [2][ncr][ncr][2] per-cell bounds of masked candidates
+  uint64_t *cbnd;    // intervis: [2][ncr][ncr][4] per-cell bounds of masked candidates
   int n, roi, nr, nwpr, cw, nsheds, intervis;
   int cs, ncr, Q, nunits, G, upw, kpu, nbuckets, cap;
   // dynamic LDS layout (byte offsets)

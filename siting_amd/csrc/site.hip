@@ -814,8 +814,9 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   // intervis multi-pick: the bound record of owned cell lc in buffer buf,
   // over the cell's unused candidates whose last area was masked (U bit 2;
   // only those can gain area, when a pick uncovers their cell -- the
-  // others' areas only shrink): (largest Bd << 32 | its position, second
-  // largest Bd << 32 | the largest's candidate)
+  // others' areas only shrink): the two largest bounds Bd with their
+  // candidates' positions and ids, and the third largest bound --
+  // (b1 << 32 | pos1, b2 << 32 | id1, b3 << 32 | pos2, id2)
   auto cell_record = [&](int lc, int buf) {
     const int ul = lc / kpu, k = lc - ul * kpu;
     const int u = ul * G + g;
@@ -823,22 +824,32 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     const int r = u / Q, cc = (u - r * Q) + k * Q;
     if (cc >= a.ncr) return;
     const int lo = OFF(ul, k), hi = OFF(ul, k + 1);
-    uint32_t b1 = 0, b2 = 0, p1 = 0, id1 = 0;
+    uint32_t b1 = 0, b2 = 0, b3 = 0, p1 = 0, p2 = 0, id1 = 0, id2 = 0;
     for (int i = lo; i < hi; i++) {
       if ((U[i] & 5) != 4) continue;
       const uint32_t b = (uint32_t)Bd[i];
       if (b > b1) {
+        b3 = b2;
         b2 = b1;
+        p2 = p1;
+        id2 = id1;
         b1 = b;
         p1 = P[i];
         id1 = (uint32_t)C[i];
       } else if (b > b2) {
+        b3 = b2;
         b2 = b;
+        p2 = P[i];
+        id2 = (uint32_t)C[i];
+      } else if (b > b3) {
+        b3 = b;
       }
     }
-    uint64_t *rec = a.cbnd + ((size_t)buf * a.ncr * a.ncr + (size_t)r * a.ncr + cc) * 2;
+    uint64_t *rec = a.cbnd + ((size_t)buf * a.ncr * a.ncr + (size_t)r * a.ncr + cc) * 4;
     st_dev(&rec[0], ((uint64_t)b1 << 32) | p1);
     st_dev(&rec[1], ((uint64_t)b2 << 32) | id1);
+    st_dev(&rec[2], ((uint64_t)b3 << 32) | p2);
+    st_dev(&rec[3], (uint64_t)id2);
   };
   // lists owned candidate (cx, cy)'s cell as changed this round
   auto mark_cell = [&](int cx, int cy) {
@@ -993,13 +1004,14 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         // intervis: a masked candidate gains area only if a pick's
         // viewshed covers its cell, so only masked candidates within roi of
         // a pick count.  From the last round's records of the cells within
-        // roi of each pick taken: every cell's second largest masked bound
-        // (its largest, when the pick does not see that candidate's cell,
-        // stays masked) gives `bound`; a cell's largest, when the pick sees
-        // it, is listed (ev_*) and computed exactly against cum and the
-        // pick's bits only when it could beat the key at hand (its area
-        // then is at most that: other picks only cover more).  Picks are
-        // never masked (a masked area is 0).
+        // roi of each pick taken: every cell's third largest masked bound
+        // (its second largest, beyond roi 40) gives `bound`; each of its two
+        // largest (its largest, beyond roi 40), when the pick sees its cell,
+        // is listed (ev_*) and computed exactly against cum and the pick's
+        // bits only when it could beat the key at hand (its area then is at
+        // most that: other picks only cover more), and when the pick does
+        // not see its cell it stays masked.  Picks are never masked (a
+        // masked area is 0).
         // Area of candidate cid (at cpos) against cum | the picks of the
         // lanes in `picks`, computed with the helpers.
         int mb_n = 0;
@@ -1055,45 +1067,55 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           const int r0 = max(0, px - roi) / a.cs, r1 = min(n - 1, px + roi) / a.cs;
           const int c0 = max(0, py - roi) / a.cs, c1 = min(n - 1, py + roi) / a.cs;
           const int ncw = c1 - c0 + 1, cnt_w = (r1 - r0 + 1) * ncw;
-          const uint64_t *cb = a.cbnd + (size_t)parity * a.ncr * a.ncr * 2;
+          const uint64_t *cb = a.cbnd + (size_t)parity * a.ncr * a.ncr * 4;
+          // a cell's second largest is listed too for roi up to 40 (a 7594^2
+          // roi-25 request: 26575 -> 17249 rounds, SITE 647 -> 573 ms), else
+          // its bound counts (the 2048^2 roi-80 job: 113 vs 118 ms listed)
+          const bool two = roi <= 40;
+          // does the pick see the cell at position pos (within its disk)?
+          // every load unconditional, from the pick's centre when outside
+          auto sees = [&](uint32_t pos, bool live) -> bool {
+            const int dx = (int)(pos >> 16) - px, dy = (int)(pos & 0xffffu) - py;
+            const bool in = live && dx * dx + dy * dy <= roi * roi;
+            const int row = in ? dx + roi : roi, col = in ? dy + roi : roi;
+            const uint64_t w = psh[(size_t)row * nwpr + (col >> 6)];
+            return in && ((w >> (63 - (col & 63))) & 1u);
+          };
           uint32_t m = 0;
-          for (int e0 = 0; e0 < cnt_w; e0 += kWave) {
-            const int e = e0 + lane;
-            bool seen_by_pick = false;
-            uint64_t w0 = 0, w1 = 0;
-            if (e < cnt_w) {
-              const int r = r0 + e / ncw, cc = c0 + e % ncw;
-              const uint64_t *rec = cb + ((size_t)r * a.ncr + cc) * 2;
-              w0 = ld_dev(&rec[0]);
-              if ((w0 >> 32) != 0) {
-                w1 = ld_dev(&rec[1]);
-                const int dx = (int)((uint32_t)w0 >> 16) - px;
-                const int dy = (int)((uint32_t)w0 & 0xffffu) - py;
-                if (dx * dx + dy * dy <= roi * roi) {
-                  const int row = dx + roi, col = dy + roi;
-                  seen_by_pick = (psh[(size_t)row * nwpr + (col >> 6)] >> (63 - (col & 63))) & 1u;
-                }
-                const uint32_t b2 = (uint32_t)(w1 >> 32);
-                m = b2 > m ? b2 : m;
-              }
-            }
-            // list the seen ones (beyond the list, their bounds count)
-            const uint64_t sv = __ballot(seen_by_pick);
+          // list a seen candidate (beyond the list, its bound counts)
+          auto list = [&](bool seen, uint32_t pos, uint32_t id, uint32_t bd) {
+            const uint64_t sv = __ballot(seen);
             const int at = ev_n + (int)__builtin_amdgcn_mbcnt_hi(
-                                       (uint32_t)(sv >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)sv, 0));
-            if (seen_by_pick) {
+                                      (uint32_t)(sv >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)sv, 0));
+            if (seen) {
               if (at < kMaxBatch) {
-                ev_pos[at] = (uint32_t)w0;
-                ev_id[at] = (uint32_t)w1;
-                ev_bd[at] = (int)(w0 >> 32);
+                ev_pos[at] = pos;
+                ev_id[at] = id;
+                ev_bd[at] = (int)bd;
                 ev_pick[at] = j;
                 ev_exact[at] = -1;
               } else {
-                const uint32_t b1 = (uint32_t)(w0 >> 32);
-                m = b1 > m ? b1 : m;
+                m = bd > m ? bd : m;
               }
             }
             ev_n += __popcll(sv);
+          };
+          for (int e0 = 0; e0 < cnt_w; e0 += kWave) {
+            const int e = e0 + lane;
+            const bool ok = e < cnt_w;
+            const int r = r0 + e / ncw, cc = c0 + e % ncw;
+            const uint64_t *rec = cb + (ok ? (size_t)r * a.ncr + cc : 0) * 4;
+            const uint64_t w0 = ld_dev(&rec[0]), w1 = ld_dev(&rec[1]);
+            const uint64_t w2 = ld_dev(&rec[2]), w3 = ld_dev(&rec[3]);
+            const uint32_t b1 = ok ? (uint32_t)(w0 >> 32) : 0, b2 = ok ? (uint32_t)(w1 >> 32) : 0;
+            const uint32_t b3 = ok ? (uint32_t)(w2 >> 32) : 0;
+            m = b3 > m ? b3 : m;  // the rest of the cell's masked candidates
+            // the two largest: only if the pick sees its cell can it gain
+            const bool s1 = sees((uint32_t)w0, b1 != 0);
+            const bool s2 = sees((uint32_t)w2, two && b2 != 0);
+            if (!two) m = b2 > m ? b2 : m;
+            list(s1, (uint32_t)w0, (uint32_t)w1, b1);
+            list(s2, (uint32_t)w2, (uint32_t)w3, b2);
           }
           if (ev_n > kMaxBatch) ev_n = kMaxBatch;
           const uint32_t wm = (uint32_t)wave_max_u64_dpp((uint64_t)m);
@@ -1941,7 +1963,7 @@ size_t scratch_layout(const Plan &p, int nsheds, uint8_t *base, Scratch *out) {
   s.phash = (uint64_t *)take(8 * 2 * (size_t)kMaxGroups);
   if (p.iv) {
     s.sB = (int32_t *)take(4 * (size_t)nsheds);
-    s.cbnd = (uint64_t *)take(8 * 2 * 2 * (size_t)p.ncr * p.ncr);
+    s.cbnd = (uint64_t *)take(8 * 2 * 4 * (size_t)p.ncr * p.ncr);
   }
   if (out) *out = s;
   return o;
