@@ -96,7 +96,10 @@ constexpr int kMaxDirtyCells = 1024;
 // Candidates the pick walk may re-key in a round (intervis 0).
 constexpr int kMaxRekey = 8;
 static_assert(kMaxBatch <= kWave, "lane j of the walk holds pick j");
-constexpr uint32_t kPickLast = 0x8000u;
+// Position of a bound entry in a lane's key list: a key that stands for
+// keys the lane does not hold (coordinates are below 32767, so no real
+// position).
+constexpr uint32_t kGhost = 0xffffffffu;
 
 
 __device__ __forceinline__ uint64_t cand_key(int area, int c) {
@@ -620,9 +623,12 @@ __device__ __forceinline__ uint64_t wave_max_u64_dpp(uint64_t v) {
 struct Top3 {
   uint64_t k[3];
   uint32_t p[3];
+  uint64_t over;  // the largest key pushed and not kept
 };
 
 __device__ __forceinline__ void top3_push(Top3 &t, uint64_t k, uint32_t p) {
+  const uint64_t out = k > t.k[2] ? t.k[2] : k;  // the one that falls off
+  t.over = out > t.over ? out : t.over;
   if (k > t.k[2]) {
     if (k > t.k[1]) {
       t.k[2] = t.k[1];
@@ -643,22 +649,38 @@ __device__ __forceinline__ void top3_push(Top3 &t, uint64_t k, uint32_t p) {
   }
 }
 
-// Best and second best (key, position) over the unused candidates of owned
-// unit ul; one wavefront.
+// Drops a lane's first key: the others move up, and the largest key the
+// lane did not keep follows them as a bound entry.
+__device__ __forceinline__ void lane_pop(Top3 &t) {
+  t.k[0] = t.k[1];
+  t.p[0] = t.p[1];
+  t.k[1] = t.k[2];
+  t.p[1] = t.p[2];
+  t.k[2] = t.over;
+  t.p[2] = t.over ? kGhost : 0;
+  t.over = 0;
+}
+
+// Best and second best (key, position) and the third best key over the
+// unused candidates of owned unit ul; one wavefront.
 struct Top2 {
-  uint64_t k1, k2;
+  uint64_t k1, k2, k3;
   uint32_t p1, p2;
 };
 
 __device__ __forceinline__ void top2_push(Top2 &t, uint64_t k, uint32_t p) {
   if (k > t.k1) {
+    t.k3 = t.k2;
     t.k2 = t.k1;
     t.p2 = t.p1;
     t.k1 = k;
     t.p1 = p;
   } else if (k > t.k2) {
+    t.k3 = t.k2;
     t.k2 = k;
     t.p2 = p;
+  } else if (k > t.k3) {
+    t.k3 = k;
   }
 }
 
@@ -668,10 +690,12 @@ __device__ __forceinline__ Top2 wave_top2(Top2 t) {
     Top2 u;
     u.k1 = __shfl_xor(t.k1, o, kWave);
     u.k2 = __shfl_xor(t.k2, o, kWave);
+    u.k3 = __shfl_xor(t.k3, o, kWave);
     u.p1 = __shfl_xor(t.p1, o, kWave);
     u.p2 = __shfl_xor(t.p2, o, kWave);
     top2_push(t, u.k1, u.p1);
     top2_push(t, u.k2, u.p2);
+    top2_push(t, u.k3, 0);
   }
   return t;
 }
@@ -709,6 +733,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   __shared__ int ev_bd[kMaxBatch], ev_pick[kMaxBatch], ev_exact[kMaxBatch];
   __shared__ uint64_t xk[3 * (kPickWaves - 1)][kWave];  // pick lists of wavefronts 1..
   __shared__ uint32_t xp[3 * (kPickWaves - 1)][kWave];
+  __shared__ uint64_t xo[kPickWaves - 1][kWave];  // their largest keys not kept
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wid = tid >> 6;
@@ -723,7 +748,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   const bool hybrid = !a.intervis && a.tau_shift > 0;
 
   uint64_t *delta = (uint64_t *)(smem + a.o_delta);
-  uint64_t *ukey = (uint64_t *)(smem + a.o_ukey);   // [upw][2]
+  uint64_t *ukey = (uint64_t *)(smem + a.o_ukey);   // [upw][3]
   uint32_t *upos = (uint32_t *)(smem + a.o_upos);   // [upw][2]
   uint8_t *dirty = smem + a.o_dirty;                 // [upw]
   int32_t *off_l = (int32_t *)(smem + a.o_off);
@@ -788,7 +813,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   // (positions are below 2^31).
   auto unit_top2 = [&](int ul) {
     const int lo = OFF(ul, 0), hi = OFF(ul, kpu);
-    Top2 t{0, 0, 0, 0};
+    Top2 t{0, 0, 0, 0, 0};
     // two candidates per lane in flight, every load unconditional (a load
     // under a branch is waited for inside it); a lane past the end re-reads
     // the unit's first candidate and pushes nothing (key 0)
@@ -805,8 +830,9 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     }
     t = wave_top2(t);
     if (lane == 0) {
-      ukey[2 * ul] = t.k1;
-      ukey[2 * ul + 1] = t.k2;
+      ukey[3 * ul] = t.k1;
+      ukey[3 * ul + 1] = t.k2;
+      ukey[3 * ul + 2] = t.k3;
       upos[2 * ul] = t.p1;
       upos[2 * ul + 1] = t.p2;
     }
@@ -889,13 +915,14 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     }
   };
   auto publish = [&](int parity) {
-    uint64_t *kn = a.keys + (size_t)parity * nunits * 2;
+    uint64_t *kn = a.keys + (size_t)parity * nunits * 3;
     uint32_t *pn = a.kpos + (size_t)parity * nunits * 2;
-    for (int e = tid; e < 2 * upw; e += kLoopThreads) {
-      const int u = (e >> 1) * G + g;
+    for (int e = tid; e < 3 * upw; e += kLoopThreads) {
+      const int ul = e / 3, x = e - ul * 3;
+      const int u = ul * G + g;
       if (u < nunits) {
-        st_dev(&kn[2 * u + (e & 1)], ukey[e]);
-        st_dev(&pn[2 * u + (e & 1)], upos[e]);
+        st_dev(&kn[3 * u + x], ukey[e]);
+        if (x < 2) st_dev(&pn[2 * u + x], upos[2 * ul + x]);
       }
     }
   };
@@ -950,17 +977,18 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     // earlier pick of the round is exact; a nearer one is re-keyed to its
     // exact area after those picks (a positive key's area only shrinks, so
     // it goes back into the order); under intervis a key is taken only if
-    // the cell bounds of the masked candidates admit it.  A unit's second key sorts below its first, so
-    // the order needs no per-unit state; taking a unit's second key ends
-    // the round, since its third is unknown.
+    // the cell bounds of the masked candidates admit it.  Each unit
+    // publishes its two largest keys with positions and its third largest
+    // key, which stands for all the unit's keys below its second (a bound
+    // entry: it ends the round if it comes first).
     //
-    // Lane l of wavefronts 0..3 keeps the three largest keys of units
-    // l, l + 64, ... (strided by 256) in registers; wavefronts 1..3 hand
-    // theirs to wavefront 0 through LDS, whose lanes then hold the three
-    // largest of units l + 64 j.  A lane's third key is flagged like a
-    // second key: taking it ends the prefix, as the lane's fourth is unknown.
+    // Lane l of wavefronts 0..3 keeps the three largest entries of units
+    // l, l + 64, ... (strided by 256) in registers, and the largest it
+    // dropped; wavefronts 1..3 hand theirs to wavefront 0 through LDS, whose
+    // lanes then hold the three largest of units l + 64 j.  Once a lane's
+    // three are taken, its largest dropped key follows as a bound entry.
     {
-      const uint64_t *kk = a.keys + (size_t)parity * nunits * 2;
+      const uint64_t *kk = a.keys + (size_t)parity * nunits * 3;
       const uint32_t *kp = a.kpos + (size_t)parity * nunits * 2;
       Top3 h{};
       if (wid < kPickWaves) {
@@ -970,16 +998,22 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           // every load unconditional (a load under a branch is waited for
           // inside it); a missing second unit reads the first's and drops it
           const int u1s = has1 ? u1 : u0;
-          const uint64_t a0 = ld_dev(&kk[2 * u0]), a1 = ld_dev(&kk[2 * u0 + 1]);
+          const uint64_t a0 = ld_dev(&kk[3 * u0]), a1 = ld_dev(&kk[3 * u0 + 1]);
+          const uint64_t a2 = ld_dev(&kk[3 * u0 + 2]);
           const uint32_t q0 = ld_dev(&kp[2 * u0]), q1 = ld_dev(&kp[2 * u0 + 1]);
-          const uint64_t b0v = ld_dev(&kk[2 * u1s]), b1v = ld_dev(&kk[2 * u1s + 1]);
+          const uint64_t b0v = ld_dev(&kk[3 * u1s]), b1v = ld_dev(&kk[3 * u1s + 1]);
+          const uint64_t b2v = ld_dev(&kk[3 * u1s + 2]);
           const uint32_t r0v = ld_dev(&kp[2 * u1s]), r1v = ld_dev(&kp[2 * u1s + 1]);
-          const uint64_t b0 = has1 ? b0v : 0, b1 = has1 ? b1v : 0;
+          const uint64_t b0 = has1 ? b0v : 0, b1 = has1 ? b1v : 0, b2 = has1 ? b2v : 0;
           const uint32_t r0 = has1 ? r0v : 0, r1 = has1 ? r1v : 0;
+          // a unit's third key stands for every key it holds below its
+          // second: a bound entry
           top3_push(h, a0, q0);
-          top3_push(h, a1, q1 | kPickLast);
+          top3_push(h, a1, q1);
+          top3_push(h, a2, kGhost);
           top3_push(h, b0, r0);
-          top3_push(h, b1, r1 | kPickLast);
+          top3_push(h, b1, r1);
+          top3_push(h, b2, kGhost);
         }
         if (wid > 0) {
           const int s = (wid - 1) * 3;
@@ -989,13 +1023,15 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           xp[s][lane] = h.p[0];
           xp[s + 1][lane] = h.p[1];
           xp[s + 2][lane] = h.p[2];
+          xo[wid - 1][lane] = h.over;
         }
       }
       __syncthreads();
       if (wid == 0) {
 #pragma unroll
         for (int s = 0; s < 3 * (kPickWaves - 1); s++) top3_push(h, xk[s][lane], xp[s][lane]);
-        if (h.k[2] != 0) h.p[2] |= kPickLast;
+#pragma unroll
+        for (int s = 0; s < kPickWaves - 1; s++) h.over = xo[s][lane] > h.over ? xo[s][lane] : h.over;
         // Lane j < k holds pick j.
         uint64_t pk = 0;
         uint32_t pp = 0;
@@ -1227,8 +1263,10 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           // the lane that holds it (keys are distinct)
           const int src = (int)__builtin_ctzll(__ballot(h.k[0] == bv));
           const uint32_t pos = (uint32_t)__builtin_amdgcn_readlane((int)h.p[0], src);
-          if (pos >> 31) break;           // stale: its area is only a bound
-          const int px = (int)(pos >> 16), py = (int)(pos & 0x7fffu);
+          // a bound entry (keys the lane does not hold may come next), or a
+          // stale key (intervis 0: its area is only a bound)
+          if (pos >> 31) break;
+          const int px = (int)(pos >> 16), py = (int)(pos & 0xffffu);
           bool near = false;
           if (lane < k) {
             const int ex = px - (int)(pp >> 16);
@@ -1239,28 +1277,15 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           if (nearp) {
             if (rn == kMaxRekey || nevals >= a.near_evals) break;
             nevals++;
-            const int area = rekey_area(~(uint32_t)bv, pos & ~kPickLast, nearp);
+            const int area = rekey_area(~(uint32_t)bv, pos, nearp);
             if (lane == 0) {
               r_key[rn] = ((uint64_t)(uint32_t)area << 32) | (uint32_t)bv;
-              r_pos[rn] = pos & ~kPickLast;
+              r_pos[rn] = pos;
               r_mark[rn] = k;
             }
             __builtin_amdgcn_wave_barrier();
             rn++;
-            if (lane == src) {
-              if (pos & kPickLast) {
-                // the lane's next keys are unknown but below this one: it
-                // stays as a bound that ends the walk if it comes first
-                h.p[0] = pos | 0x80000000u;
-              } else {
-                h.k[0] = h.k[1];
-                h.p[0] = h.p[1];
-                h.k[1] = h.k[2];
-                h.p[1] = h.p[2];
-                h.k[2] = 0;
-                h.p[2] = 0;
-              }
-            }
+            if (lane == src) lane_pop(h);
             continue;
           }
           if (ivm && k > 0) {
@@ -1272,18 +1297,10 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           }
           if (lane == k) {
             pk = bv;
-            pp = pos & ~kPickLast;
+            pp = pos;
           }
-          if (lane == src) {
-            h.k[0] = h.k[1];
-            h.p[0] = h.p[1];
-            h.k[1] = h.k[2];
-            h.p[1] = h.p[2];
-            h.k[2] = 0;
-            h.p[2] = 0;
-          }
+          if (lane == src) lane_pop(h);
           k++;
-          if (pos & kPickLast) break;     // the next key of its unit or lane is unknown
         }
         if (lane < k) {
           bw_key[lane] = pk;
@@ -1879,7 +1896,7 @@ Plan make_plan(int n, int roi, int nsheds, int groups_cap, int fixed_groups = 0,
   p.o_delta = (int)o;
   if (p.delta_lds) o += delta_bytes;
   p.o_ukey = (int)o;
-  o += sizeof(uint64_t) * 2 * (size_t)p.upw;
+  o += sizeof(uint64_t) * 3 * (size_t)p.upw;
   p.o_upos = (int)o;
   o += sizeof(uint32_t) * 2 * (size_t)p.upw;
   p.o_dirty = (int)o;
@@ -1958,7 +1975,7 @@ size_t scratch_layout(const Plan &p, int nsheds, uint8_t *base, Scratch *out) {
   s.nb = (uint64_t *)take(8 * p.nbwords);
   // global-mode seen bitmaps: one slice of ceil(nsheds/32)+1 words per group
   s.sseen = (unsigned int *)take(4 * ((size_t)nsheds / 32 + 2) * kMaxGroups);
-  s.keys = (uint64_t *)take(8 * 4 * (size_t)p.nunits);
+  s.keys = (uint64_t *)take(8 * 6 * (size_t)p.nunits);
   s.kpos = (uint32_t *)take(4 * 4 * (size_t)p.nunits);
   s.phash = (uint64_t *)take(8 * 2 * (size_t)kMaxGroups);
   if (p.iv) {
@@ -2022,12 +2039,12 @@ siting_status site_make_args(const Geometry &g, const int32_t *obs,
     // intervis rounds of several picks (SITING_SITE_IV_MULTI=0: one per round)
     const char *e = std::getenv("SITING_SITE_IV_MULTI");
     a.iv_multi = g.p.intervis && !(e && *e && std::atoi(e) == 0);
-    // rule: 8 for roi up to 40 (measured: a 7594^2 roi-25 request's SITE
-    // 816 -> 659 ms) and above 75 (the tuner's roi-100 to 200 classes:
-    // 5-12 % faster), none between (a 7857^2 roi-50 request: 231 ms with
-    // none, 282-389 with 2-16); a job's tuning replaces the rule,
-    // SITING_SITE_WALK_EVALS overrides both
-    int ev = (roi <= 40 || roi > 75) ? 8 : 0;
+    // rule: 16 for roi up to 40 (measured on a 7594^2 roi-25 request: SITE
+    // 850, 614 and 555 ms with 2, 8 and 16), 8 above 75 (the tuner's roi-100
+    // to 200 classes: 5-12 % faster than none), none between (a 7857^2
+    // roi-50 request: 255 ms with none, 310-416 with 2-16); a job's tuning
+    // replaces the rule, SITING_SITE_WALK_EVALS overrides both
+    int ev = roi <= 40 ? 16 : roi > 75 ? 8 : 0;
     if (g.t.site_walk_evals >= 1 && g.t.site_walk_evals <= kMaxBatch + 1)
       ev = g.t.site_walk_evals - 1;
     const char *w = std::getenv("SITING_SITE_WALK_EVALS");

@@ -14,7 +14,7 @@ own knobs, the others held at the rules:
             bitsets, one LDS bitset, HBM; the modes whose bitsets fit)
   SITE      column classes 2, 3, times stale threshold shift 4, 5, 6;
             and, on the same terrain with intervis 1, the pick walk's exact
-            area computations per round: none, 2 or 8
+            area computations per round: none, 2, 8 or 16
 
 A choice replaces the rule's only when it is at least 3 % faster.  The job
 is then run again with the chosen values, and its vix, observers, viewshed
@@ -92,7 +92,7 @@ def rule_choice(roi: int, ntests: int, n: int, nsheds: int) -> dict:
     return {"vix_halo": 32 if roi >= 64 and ntests >= 64 else 16,
             "viewshed_threads": 256, "viewshed_mode": modes[0],
             "site_q": 3 if q_list >= 3000.0 else 2, "site_tau_shift": 5,
-            "site_walk_evals": 9 if roi <= 40 or roi > 75 else 1}
+            "site_walk_evals": 17 if roi <= 40 else 9 if roi > 75 else 1}
 
 
 def tune_class(key: str, roi: int, ntests: int, edge: int, reps: int, log):
@@ -159,7 +159,7 @@ def tune_class(key: str, roi: int, ntests: int, edge: int, reps: int, log):
         job.sync()
         ref_iv = _outputs(job)
         te = {}
-        for ev in (1, 3, 9):
+        for ev in (1, 3, 9, 17):
             job.set_tuning(sa.Tuning(**dict(choice, site_walk_evals=ev)))
             te[ev] = _time_stage(job, "site", max(1, reps - 1))
         choice["site_walk_evals"] = _pick(te, rule["site_walk_evals"])
