@@ -18,6 +18,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libsiting_amd.so")
+# the same library with the greedy loop's phase counters (siting_amd.phases)
+PROF_LIB_PATH = os.path.join(_HERE, "lib", "libsiting_amd_prof.so")
 CLI_PATH = os.path.join(_HERE, "bin", "hipsite")
 
 OK, EINVAL, ERANGE, ESMALLBLOCK, ENOMEM, EDEVICE, EIO, EBUSY, ENOTFOUND = range(9)
@@ -171,6 +173,19 @@ def lib() -> ctypes.CDLL:
             fn.argtypes = args
         _lib = handle
     return _lib
+
+
+def use_profiling_build() -> ctypes.CDLL:
+    """Loads the profiling build in place of the product library; must come
+    before anything else loads the library in this process."""
+    global LIB_PATH
+    if _lib is not None and LIB_PATH != PROF_LIB_PATH:
+        raise SitingError(EINVAL, "the product library is already loaded")
+    LIB_PATH = PROF_LIB_PATH
+    handle = lib()
+    handle.siting_phase_profile_read.restype = _I
+    handle.siting_phase_profile_read.argtypes = [_VP, ctypes.c_int, ctypes.c_int]
+    return handle
 
 
 def _check(status: int) -> None:

@@ -71,6 +71,26 @@
 
 namespace siting {
 
+// Phase profile of the greedy loop (the profiling build only, compiled with
+// SITING_PHASE_PROFILE; the product build has none of it).  Thread 0 of every
+// loop workgroup adds the device-clock time (100 MHz) of each phase of each
+// round; the walking wavefront adds the time spent in its exact area
+// computations, its steps, and why each round's walk ended.  Counters
+// (g_site_prof, summed over workgroups and rounds):
+//   0 pick walk (after the key gather), 1 bookkeeping and the walk barrier,
+//   2 union, 3 update lists, 4 re-evaluation, 5 top-2 and publish,
+//   6 end-of-round barrier, 7 key gather, 8 loop workgroups,
+//   9 exact computations, 10 their time, 11 walk steps, 12 picks,
+//   13-19 walk ends: computation budget, rekeyed set full, bound entry,
+//   stale key, no area left, pick cap, round without a pick.
+#ifdef SITING_PHASE_PROFILE
+constexpr int kProfWords = 32;
+__device__ unsigned long long g_site_prof[kProfWords];
+#define SITE_PROF(stmt) stmt
+#else
+#define SITE_PROF(stmt)
+#endif
+
 namespace {
 
 constexpr int kScanThreads = 1024;  // the bucketing scan kernels
@@ -703,6 +723,20 @@ __device__ __forceinline__ Top2 wave_top2(Top2 t) {
 // The greedy loop of one job, run by workgroups g = 0..G-1 (all resident).
 __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) {
   extern __shared__ __align__(16) uint8_t smem[];
+#ifdef SITING_PHASE_PROFILE
+  __shared__ unsigned long long pf_acc[kProfWords], pf_last;
+#define SITE_PROF_MARK(k)                                  \
+  if (threadIdx.x == 0) {                                  \
+    const unsigned long long t_ = wall_clock64();          \
+    pf_acc[k] += t_ - pf_last;                             \
+    pf_last = t_;                                          \
+  }
+#define SITE_PROF_ADD(k, v) \
+  if ((threadIdx.x & 63) == 0) pf_acc[k] += (unsigned long long)(v);
+#else
+#define SITE_PROF_MARK(k)
+#define SITE_PROF_ADD(k, v)
+#endif
   __shared__ int list_i[kMaxList];
   __shared__ int list_acc[kMaxList];
   __shared__ int list_n;
@@ -959,7 +993,12 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   const bool walk_help =
       (a.intervis ? ivm && (a.walk_evals > 0 || a.near_evals > 0) : a.near_evals > 0) &&
                          (2 * roi + 1) * (nwpr + 1) > kWave * kBatch;
+  SITE_PROF(if (tid == 0) {
+    for (int q = 0; q < kProfWords; q++) pf_acc[q] = 0;
+    pf_last = wall_clock64();
+  })
   while (started) {
+    SITE_PROF_MARK(6)
     if (tid == 0) mb_seq = 0;
     if (wid == kLoopWaves - 1) {
       bool bad = false;
@@ -1027,6 +1066,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         }
       }
       __syncthreads();
+      SITE_PROF_MARK(7)
       if (wid == 0) {
 #pragma unroll
         for (int s = 0; s < 3 * (kPickWaves - 1); s++) top3_push(h, xk[s][lane], xp[s][lane]);
@@ -1051,7 +1091,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         // Area of candidate cid (at cpos) against cum | the picks of the
         // lanes in `picks`, computed with the helpers.
         int mb_n = 0;
-        auto walk_area = [&](uint32_t cid, uint32_t cpos, uint64_t picks) -> int {
+        auto walk_area_core = [&](uint32_t cid, uint32_t cpos, uint64_t picks) -> int {
           if (!walk_help) {  // a small shed: this wavefront alone
             const int np = __popcll(picks);
             const int slot = (int)__builtin_amdgcn_mbcnt_hi(
@@ -1092,6 +1132,13 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
               __builtin_amdgcn_s_sleep(1);
           acc = __shfl(acc, 0, kWave);
           return own + (int)(acc & ((1ull << 40) - 1));
+        };
+        auto walk_area = [&](uint32_t cid, uint32_t cpos, uint64_t picks) -> int {
+          SITE_PROF(const unsigned long long t0 = wall_clock64();)
+          const int r = walk_area_core(cid, cpos, picks);
+          SITE_PROF_ADD(10, wall_clock64() - t0)
+          SITE_PROF_ADD(9, 1)
+          return r;
         };
         uint32_t bound = 0;
         int nwin = 0, ev_n = 0;
@@ -1208,6 +1255,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           return walk_area(cid, cpos, nearp);
         };
         while (k < kmax) {
+          SITE_PROF_ADD(11, 1)
           const uint64_t bv = wave_max_u64_dpp(h.k[0]);
           if (rn > 0) {
             const uint64_t rkv = lane < rn ? r_key[lane] : 0;
@@ -1226,7 +1274,10 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
               }
               const uint64_t nearp = __ballot(near_any);
               if (__ballot(fresh)) {  // picks since its last computation reach it
-                if (nevals >= a.near_evals) break;
+                if (nevals >= a.near_evals) {
+                  SITE_PROF_ADD(13, 1)
+                  break;
+                }
                 nevals++;
                 const int area = rekey_area(~(uint32_t)rv, rpos, nearp);
                 if (lane == 0) {
@@ -1236,7 +1287,10 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
                 __builtin_amdgcn_wave_barrier();
                 continue;
               }
-              if ((rv >> 32) == 0) break;  // no candidate adds area
+              if ((rv >> 32) == 0) {  // no candidate adds area
+                SITE_PROF_ADD(17, 1)
+                break;
+              }
               if (ivm) {  // it must beat every masked candidate's bound, as below
                 for (; nwin < k; nwin++) window_scan(nwin);
                 if ((uint32_t)(rv >> 32) <= bound || listed_block((int)(rv >> 32))) break;
@@ -1258,14 +1312,20 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
               continue;
             }
           }
-          if ((bv >> 32) == 0) break;     // no candidate adds area
+          if ((bv >> 32) == 0) {  // no candidate adds area
+            SITE_PROF_ADD(17, 1)
+            break;
+          }
           if (k == 0) head = (int)(bv >> 32);
           // the lane that holds it (keys are distinct)
           const int src = (int)__builtin_ctzll(__ballot(h.k[0] == bv));
           const uint32_t pos = (uint32_t)__builtin_amdgcn_readlane((int)h.p[0], src);
           // a bound entry (keys the lane does not hold may come next), or a
           // stale key (intervis 0: its area is only a bound)
-          if (pos >> 31) break;
+          if (pos >> 31) {
+            SITE_PROF_ADD(pos == kGhost ? 15 : 16, 1)
+            break;
+          }
           const int px = (int)(pos >> 16), py = (int)(pos & 0xffffu);
           bool near = false;
           if (lane < k) {
@@ -1275,7 +1335,10 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           }
           const uint64_t nearp = __ballot(near);
           if (nearp) {
-            if (rn == kMaxRekey || nevals >= a.near_evals) break;
+            if (rn == kMaxRekey || nevals >= a.near_evals) {
+              SITE_PROF_ADD(rn == kMaxRekey ? 14 : 13, 1)
+              break;
+            }
             nevals++;
             const int area = rekey_area(~(uint32_t)bv, pos, nearp);
             if (lane == 0) {
@@ -1302,6 +1365,9 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           if (lane == src) lane_pop(h);
           k++;
         }
+        SITE_PROF_ADD(12, k)
+        SITE_PROF_ADD(18, k >= kmax)
+        SITE_PROF_ADD(19, k == 0)
         if (lane < k) {
           bw_key[lane] = pk;
           bw_pos[lane] = pp;
@@ -1337,6 +1403,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       niter = -2;
       break;
     }
+    SITE_PROF_MARK(0)
     int k = bw_n;
     // no pick while the largest key is stale: the round only re-evaluates
     const bool eval_only = k == 0 && bw_head > 0;
@@ -1432,6 +1499,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       break;
     }
 
+    SITE_PROF_MARK(1)
     // ---- union of the picks into cum (cpp:393-419)
     Winner W;
     {
@@ -1519,6 +1587,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     // 2*roi window (unit/cell offsets); their concatenation is walked in
     // windows of kMaxList items; a candidate in several ranges is listed
     // once (seen bitmap, cleared at the end of the round).
+    SITE_PROF_MARK(2)
     if (tid == 0) rng_n = 0;
     __syncthreads();
     for (int t = tid; t < kc * upw; t += kLoopThreads) {
@@ -1652,6 +1721,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       if (tid == 0) list_n = 0;
       __syncthreads();
     }
+    SITE_PROF_MARK(3)
     for (int i = tid; i < (cnt + 31) / 32; i += kLoopThreads) seen[i] = 0;
     if (stop) break;
     // before the re-evaluation reads cum (intervis rounds re-evaluate
@@ -1723,6 +1793,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       // the update walk of the next round starts from an empty list
       if (tid == 0) list_n = 0;
     }
+    SITE_PROF_MARK(4)
     for (int ul = wid; ul < upw; ul += kLoopWaves)
       if (dirty[ul]) unit_top2(ul);
     __syncthreads();
@@ -1730,11 +1801,17 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     parity ^= 1;
     publish(parity);
     if (ivm) publish_cells(parity);
+    SITE_PROF_MARK(5)
     if (!grid_sync(&a.st->barrier, bar_target, (unsigned)G, a.spin_limit)) {
       niter = -1;
       break;
     }
   }
+  SITE_PROF(if (tid == 0) {
+    pf_acc[8] = 1;
+    for (int q = 0; q < kProfWords; q++)
+      if (pf_acc[q]) atomicAdd(&g_site_prof[q], pf_acc[q]);
+  })
 
   // sum_u: each thread counted its own list entries
   {
@@ -2175,3 +2252,19 @@ siting_status site_launch_queue(const SiteArgs *jobs_dev, const int *plan_dev,
 }
 
 }  // namespace siting
+
+#ifdef SITING_PHASE_PROFILE
+// Profiling build only: copies the greedy loop's phase counters (n words,
+// at most 32, in the order listed at g_site_prof) and optionally clears them.
+extern "C" siting_status siting_phase_profile_read(unsigned long long *out, int n, int reset) {
+  if (!out || n < 1 || n > siting::kProfWords) return SITING_EINVAL;
+  SITING_HIP(hipDeviceSynchronize());
+  SITING_HIP(hipMemcpyFromSymbol(out, HIP_SYMBOL(siting::g_site_prof),
+                                 sizeof(unsigned long long) * (size_t)n));
+  if (reset) {
+    unsigned long long z[siting::kProfWords] = {0};
+    SITING_HIP(hipMemcpyToSymbol(HIP_SYMBOL(siting::g_site_prof), z, sizeof(z)));
+  }
+  return SITING_OK;
+}
+#endif
