@@ -787,6 +787,7 @@ struct siting_batch {
   size_t lds = 0;
   int slot = 16, nslots = 1, nclasses = 1, total_groups = 16, max_slots = 1;
   int budget = 16;  // workgroups of the launch (every plan stays within it)
+  int iv_mode = 2;  // the jobs' common intervis flag (0, 1), or 2 when they differ
   bool use_graph = false, replan = true;
   int runs = 0, runs_seen = 0;
   // slot tuning: plans tried so far and the current one
@@ -794,6 +795,7 @@ struct siting_batch {
   siting::QueuePlan cur;
   siting::SlotClasses cur_classes;
   std::vector<siting::TriedPlan> tried;
+  std::vector<siting::SlotClasses> pending;  // model plans still to measure
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -817,7 +819,7 @@ siting_status batch_set_plan(siting_batch *b, const QueuePlan &p,
   std::vector<int> packed;
   packed.push_back((int)p.start.size());
   packed.push_back((int)p.off.size() - 1);
-  for (const auto *v : {&p.start, &p.size, &p.cls, &p.off, &p.order})
+  for (const auto *v : {&p.start, &p.size, &p.cls, &p.off, &p.order, &p.fb})
     packed.insert(packed.end(), v->begin(), v->end());
   SITING_HIP(hipMemcpy(b->args_dev, args.data(), sizeof(SiteArgs) * (size_t)njobs,
                        hipMemcpyHostToDevice));
@@ -869,7 +871,7 @@ siting_status batch_enqueue_site(siting_batch *b) {
   hipStream_t s0 = b->streams[0];
   SITING_HIP(hipEventRecord(b->ts, s0));
   siting_status st = site_launch_queue(b->args_dev, b->plan_dev, b->nslots, b->nclasses,
-                                       b->total_groups, b->lds, b->ctl, b->jobt, s0);
+                                       b->total_groups, b->lds, b->ctl, b->jobt, b->iv_mode, s0);
   if (st != SITING_OK) return st;
   SITING_HIP(hipEventRecord(b->t1, s0));
   return SITING_OK;
@@ -905,7 +907,8 @@ void batch_drop_graph(siting_batch *b) {
 // depends on its job (rounds against per-round work) and on what else runs
 // beside it:
 //  - run 1 uses equal slots of the base size;
-//  - run 2 the mixed-slot plan of the square-root model below;
+//  - runs 2-4 the three best mixed-slot plans of the square-root model
+//    (slot_plan.hpp), measured since the model ranks them only roughly;
 //  - each later run, up to kTuneRuns, moves the job whose loop ran longest
 //    in the fastest plan measured so far onto a slot of its own twice as
 //    large (at least 32 groups), paid for with base slots; a move that does
@@ -917,7 +920,8 @@ void batch_drop_graph(siting_batch *b) {
 // (Alone, a 3365^2 roi-150 job of the uneven batch takes 84 ms on 8 groups,
 // 26 on 32 and 18 on 64; a 2593^2 roi-150 one with 2761 rounds 230, 90 and
 // 77: loops differ widely in how far more groups help.)
-constexpr int kTuneRuns = 5;
+constexpr int kTuneRuns = 7;
+constexpr int kModelPlans = 3;
 
 // After a synced run: record the plan's measured launch time and job end
 // times, and choose the next plan.
@@ -943,11 +947,21 @@ siting_status batch_tune(siting_batch *b) {
   bool have = false;
   if (b->tune_left > 0) {
     if (b->tried.size() == 1) {
-      have = sqrt_plan(ms, g,
-                       [&](int i, int size) {
-                         return (double)site_loop_groups(b->jobs[(size_t)i]->g, b->cus, size);
-                       },
-                       b->budget, b->slot, &next);
+      std::vector<SlotClasses> plans;
+      sqrt_plans(ms, g,
+                 [&](int i, int size) {
+                   return (double)site_loop_groups(b->jobs[(size_t)i]->g, b->cus, size);
+                 },
+                 b->budget, b->slot, kModelPlans, &plans);
+      b->pending.assign(plans.begin(), plans.end());
+    }
+    while (!have && !b->pending.empty()) {
+      next = b->pending.front();
+      b->pending.erase(b->pending.begin());
+      QueuePlan pp;
+      std::vector<int> pc;
+      to_plan(next, njobs, b->slot, &pp, &pc);
+      have = (int)pp.start.size() <= b->max_slots;
     }
     // local moves from the best plan: its jobs by loop time, longest first,
     // skipping the moves already tried from it
@@ -1038,6 +1052,9 @@ siting_status siting_batch_create(int device, int njobs, const siting_params *pa
   }
   SITING_HIP(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device));
   const int cus = b->cus;
+  b->iv_mode = params[0].intervis ? 1 : 0;
+  for (int i = 1; i < njobs; i++)
+    if ((params[i].intervis ? 1 : 0) != b->iv_mode) b->iv_mode = 2;
   const int nstreams = opts && opts->streams > 0 ? opts->streams : 4;
   // 8 workgroups per base slot measured best on the uneven 64-job batch
   // (step ms with re-planned slots: 4: 328, 6: 313, 8: 290, 12: 295,

@@ -284,12 +284,16 @@ int site_loop_groups(const Geometry &g, int device_cus, int groups_cap);
 // total_groups workgroups split into slots.
 //   plan (device ints, site_queue_plan_ints of them): nslots, nclasses,
 //     start[nslots] (first workgroup, increasing), size[nslots],
-//     cls[nslots], off[nclasses + 1], order[njobs]: class c's queue is
-//     order[off[c] .. off[c+1]), indices into the SiteArgs array, and every
-//     job of class c is planned for at most the size of class c's slots.
+//     cls[nslots], off[nclasses + 1], order[njobs], fb[nclasses]: class c's
+//     queue is order[off[c] .. off[c+1]), indices into the SiteArgs array,
+//     and every job of class c is planned for at most the size of class c's
+//     slots; fb[c] is a class whose jobs c's slots take once c's queue is
+//     empty (its slots no larger than c's), or -1.
 //   ctl: site_queue_ctl_words(nslots, nclasses) words of queue state.
 //   jobt: 2 words per job, its loop's start and end on the device clock
 //     (wall_clock64, 100 MHz).
+//   iv_mode: 0 or 1 when every job's intervis flag is that (the loop is
+//     compiled for it), 2 when they differ.
 siting_status site_make_args(const Geometry &g, const int32_t *obs,
                              const uint64_t *sheds, const SiteBuffers &b, int slot,
                              SiteArgs *out, size_t *lds);
@@ -298,7 +302,7 @@ size_t site_queue_ctl_words(int nslots, int nclasses);
 size_t site_queue_plan_ints(int nslots, int nclasses, int njobs);
 siting_status site_launch_queue(const SiteArgs *jobs_dev, const int *plan_dev,
                                 int nslots, int nclasses, int total_groups, size_t lds,
-                                unsigned int *ctl, unsigned long long *jobt,
+                                unsigned int *ctl, unsigned long long *jobt, int iv_mode,
                                 hipStream_t s);
 
 // host tables for VIX

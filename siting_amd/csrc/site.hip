@@ -721,7 +721,12 @@ __device__ __forceinline__ Top2 wave_top2(Top2 t) {
 }
 
 // The greedy loop of one job, run by workgroups g = 0..G-1 (all resident).
+// kIvMode: 0 or 1, the job's intervis flag fixed at compile time, so that an
+// intervis-0 loop carries none of the intervis code and its registers; 2,
+// read from a.intervis (a work queue whose jobs differ in it).
+template <int kIvMode>
 __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) {
+  const bool kIv = kIvMode == 2 ? a.intervis != 0 : kIvMode == 1;
   extern __shared__ __align__(16) uint8_t smem[];
 #ifdef SITING_PHASE_PROFILE
   __shared__ unsigned long long pf_acc[kProfWords], pf_last;
@@ -777,9 +782,9 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   const size_t S = (size_t)nr * nwpr;
   const int dw = nwpr + 1;
   // intervis rounds of several picks (else one pick per round)
-  const bool ivm = a.intervis && a.iv_multi;
-  const int kmax = a.intervis && !ivm ? 1 : kMaxBatch;
-  const bool hybrid = !a.intervis && a.tau_shift > 0;
+  const bool ivm = kIv && a.iv_multi;
+  const int kmax = kIv && !ivm ? 1 : kMaxBatch;
+  const bool hybrid = !kIv && a.tau_shift > 0;
 
   uint64_t *delta = (uint64_t *)(smem + a.o_delta);
   uint64_t *ukey = (uint64_t *)(smem + a.o_ukey);   // [upw][3]
@@ -991,7 +996,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   // (a shed box of at most kWave * kBatch items is computed by the walking
   // wavefront alone: the handoff would cost more than it saves)
   const bool walk_help =
-      (a.intervis ? ivm && (a.walk_evals > 0 || a.near_evals > 0) : a.near_evals > 0) &&
+      (kIv ? ivm && (a.walk_evals > 0 || a.near_evals > 0) : a.near_evals > 0) &&
                          (2 * roi + 1) * (nwpr + 1) > kWave * kBatch;
   SITE_PROF(if (tid == 0) {
     for (int q = 0; q < kProfWords; q++) pf_acc[q] = 0;
@@ -1238,7 +1243,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         // roi can cover it); its area can then only have shrunk, since a
         // candidate whose key is positive was not masked.
         auto rekey_area = [&](uint32_t cid, uint32_t cpos, uint64_t nearp) -> int {
-          if (a.intervis) {
+          if (kIv) {
             const int ccx = (int)(cpos >> 16), ccy = (int)(cpos & 0xffffu);
             bool cov = false;
             if ((nearp >> lane) & 1) {
@@ -1515,7 +1520,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       W.sw = a.sheds + (size_t)w * S;
       W.delta = nullptr;
     }
-    if (a.intervis && k == 1) {
+    if (kIv && k == 1) {
       // one pick: areas are recomputed against cum | delta while the union
       // is written (a reader sees either word, the OR is the same)
       if (a.delta_lds) {
@@ -1579,7 +1584,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     };
     // intervis rounds of several picks recompute their update candidates
     // against the whole union: its barrier completes first
-    const bool ivk = a.intervis && k > 1;
+    const bool ivk = kIv && k > 1;
     if (ivk) union_wait();
 
     // ---- update lists of picks 0..kc-1 among this workgroup's candidates.
@@ -1660,7 +1665,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
             dirty[((cx / a.cs) * Q + (cy / a.cs) % Q) / G] = 1;
           } else if (m) {
             int acc = m;
-            if (!stop && a.intervis) {  // cpp:452-457
+            if (!stop && kIv) {  // cpp:452-457
               uint64_t word = ld_dev(&a.cum[cum_index(cx, cy >> 6, cw)]);
               if (!ivk) {  // one pick: cum | its bits
                 const int dr = cx - W.wx + roi, dk = (cy >> 6) - W.w0w;
@@ -1685,7 +1690,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       __syncthreads();  // counts read before the areas overwrite them
       // (an intervis update reads cum | the winner's bits, the same before,
       // during and after the union: it needs no wait)
-      if (!stop && lcnt > 0 && !a.intervis) union_wait();
+      if (!stop && lcnt > 0 && !kIv) union_wait();
       if (!stop && lcnt > 0 && !failed) {
         // one wavefront per update candidate
         for (int e = wid; e < lcnt; e += kLoopWaves) {
@@ -1695,7 +1700,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           const int cx = (int)(ps >> 16), cy = (int)(ps & 0xffffu);
           const uint64_t *sh = a.sheds + (size_t)C[i] * S;
           int s;
-          if (a.intervis)
+          if (kIv)
             s = wave_new_bits(sh, cx, cy, a, W, lane, !ivk);
           else  // exact area minus the bits this round's picks covered
             s = T[i] - wave_lost_bits(sh, cx, cy, a, k, bw_pos, lane);
@@ -1726,7 +1731,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     if (stop) break;
     // before the re-evaluation reads cum (intervis rounds re-evaluate
     // nothing; the end-of-round barrier still counts every union)
-    if (!a.intervis) union_wait();
+    if (!kIv) union_wait();
     if (failed) {
       niter = -1;
       break;
@@ -1828,8 +1833,9 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   }
 }
 
+template <int kIvMode>
 __global__ __launch_bounds__(kLoopThreads) void site_loop(SiteArgs a) {
-  site_loop_body(a, blockIdx.x, gridDim.x);
+  site_loop_body<kIvMode>(a, blockIdx.x, gridDim.x);
 }
 
 // Work queue of jobs.  The grid is cut into slots of `slot` workgroups; the
@@ -1839,39 +1845,52 @@ __global__ __launch_bounds__(kLoopThreads) void site_loop(SiteArgs a) {
 // double-buffered by the slot's turn so a late reader never sees the next
 // turn's value.
 // Slots of workgroups pop jobs from their class's queue (plan layout in
-// site_queue_plan_ints) and run each job's loop with the slot's first G
-// workgroups; jobt[j] receives the loop's start and end (device clock).
+// site_queue_plan_ints), then from their class's fallback queue, and run
+// each job's loop with the slot's first G workgroups (a fallback job is
+// planned for fewer groups than the slot has); jobt[j] receives the loop's
+// start and end (device clock).
+template <int kIvMode>
 __global__ __launch_bounds__(kLoopThreads) void site_loop_queue(
     const SiteArgs *__restrict__ jobs, const int *plan, unsigned int *ctl,
     unsigned long long *jobt) {
   __shared__ int job;
   const int nslots = plan[0], ncls = plan[1];
   const int *start = plan + 2, *size = start + nslots, *cls = size + nslots;
-  const int *off = cls + nslots, *order = off + ncls + 1;
+  const int *off = cls + nslots, *order = off + ncls + 1, *fb = order + off[ncls];
   int sl = 0;  // slots are listed by first workgroup
   while (sl + 1 < nslots && (int)blockIdx.x >= start[sl + 1]) sl++;
   const int g = blockIdx.x - start[sl], slot = size[sl], c = cls[sl];
   unsigned int *queue = ctl + c;
   unsigned int *slot_job = ctl + ncls + 2 * sl;
   unsigned int *slot_bar = ctl + ncls + 2 * nslots + sl;
-  const int qlen = off[c + 1] - off[c];
+  const unsigned int qlen = (unsigned int)(off[c + 1] - off[c]);
+  const int f = fb[c];
+  constexpr unsigned int kDone = 0xffffffffu;
   unsigned int target = 0;
   for (int turn = 0;; turn++) {
     unsigned int *cell = &slot_job[turn & 1];
-    if (g == 0 && threadIdx.x == 0)
-      __hip_atomic_store(cell, atomicAdd(queue, 1u), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
+    if (g == 0 && threadIdx.x == 0) {
+      // the job's position in order[]: the next of the class's queue, else
+      // of its fallback class's queue, else none
+      const unsigned int q = atomicAdd(queue, 1u);
+      unsigned int pos = q < qlen ? (unsigned int)off[c] + q : kDone;
+      if (pos == kDone && f >= 0) {
+        const unsigned int q2 = atomicAdd(ctl + f, 1u);
+        if (q2 < (unsigned int)(off[f + 1] - off[f])) pos = (unsigned int)off[f] + q2;
+      }
+      __hip_atomic_store(cell, pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     if (!grid_sync(slot_bar, target, (unsigned)slot)) break;
     if (threadIdx.x == 0)
       job = (int)__hip_atomic_load(cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
-    const int idx = job;
-    if (idx >= qlen) break;
+    const unsigned int pos = (unsigned int)job;
+    if (pos == kDone) break;
     // a wave-uniform job index lets the job's arguments be scalar loads
-    const int j = __builtin_amdgcn_readfirstlane(order[off[c] + idx]);
+    const int j = __builtin_amdgcn_readfirstlane(order[pos]);
     const SiteArgs &a = jobs[j];
     const unsigned long long t0 = wall_clock64();
-    if (g < a.G) site_loop_body(a, g, a.G);
+    if (g < a.G) site_loop_body<kIvMode>(a, g, a.G);
     __syncthreads();
     if (g == 0 && threadIdx.x == 0) {
       jobt[2 * j] = t0;
@@ -2220,7 +2239,10 @@ siting_status launch_site(const Geometry &g, const int32_t *obs,
   siting_status st = site_make_args(g, obs, sheds, b, p.G, &a, &lds);
   if (st != SITING_OK) return st;
   if ((st = site_launch_prep(a, s)) != SITING_OK) return st;
-  hipLaunchKernelGGL(site_loop, dim3(a.G), dim3(kLoopThreads), lds, s, a);
+  if (a.intervis)
+    hipLaunchKernelGGL(site_loop<1>, dim3(a.G), dim3(kLoopThreads), lds, s, a);
+  else
+    hipLaunchKernelGGL(site_loop<0>, dim3(a.G), dim3(kLoopThreads), lds, s, a);
   SITING_HIP(hipGetLastError());
   return SITING_OK;
 }
@@ -2235,18 +2257,23 @@ size_t site_queue_ctl_words(int nslots, int nclasses) {
 }
 
 size_t site_queue_plan_ints(int nslots, int nclasses, int njobs) {
-  return 2 + 3 * (size_t)nslots + (size_t)nclasses + 1 + (size_t)njobs;
+  return 2 + 3 * (size_t)nslots + (size_t)nclasses + 1 + (size_t)njobs + (size_t)nclasses;
 }
 
 siting_status site_launch_queue(const SiteArgs *jobs_dev, const int *plan_dev,
                                 int nslots, int nclasses, int total_groups, size_t lds,
-                                unsigned int *ctl, unsigned long long *jobt,
+                                unsigned int *ctl, unsigned long long *jobt, int iv_mode,
                                 hipStream_t s) {
   const int words = (int)site_queue_ctl_words(nslots, nclasses);
   hipLaunchKernelGGL(site_zero_ctl, dim3((words + 255) / 256), dim3(256), 0, s, ctl,
                      words);
-  hipLaunchKernelGGL(site_loop_queue, dim3(total_groups), dim3(kLoopThreads), lds, s,
-                     jobs_dev, plan_dev, ctl, jobt);
+#define SITING_QUEUE(M)                                                                   \
+  hipLaunchKernelGGL(site_loop_queue<M>, dim3(total_groups), dim3(kLoopThreads), lds, s, \
+                     jobs_dev, plan_dev, ctl, jobt)
+  if (iv_mode == 0) SITING_QUEUE(0);
+  else if (iv_mode == 1) SITING_QUEUE(1);
+  else SITING_QUEUE(2);
+#undef SITING_QUEUE
   SITING_HIP(hipGetLastError());
   return SITING_OK;
 }

@@ -14,9 +14,12 @@ namespace siting {
 
 // Device form of a plan (site_loop_queue reads it; layout in common.hpp,
 // site_queue_plan_ints): slot first workgroups and sizes, each slot's queue
-// class, and the classes' job queues concatenated.
+// class, the classes' job queues concatenated, and each class's fallback:
+// the class of the smallest slots, whose jobs its slots take once their own
+// queue is empty (-1: none), so that a plan that sized its slots wrongly
+// does not leave big slots idle while small jobs wait in line.
 struct QueuePlan {
-  std::vector<int> start, size, cls, off, order;
+  std::vector<int> start, size, cls, off, order, fb;
   int total() const {
     int t = 0;
     for (int v : size) t += v;
@@ -33,7 +36,7 @@ struct SlotClasses {
 
 inline bool same_plan(const QueuePlan &x, const QueuePlan &y) {
   return x.start == y.start && x.size == y.size && x.cls == y.cls && x.off == y.off &&
-         x.order == y.order;
+         x.order == y.order && x.fb == y.fb;
 }
 
 // The device form of c; cap[i] = the slot size of job i's class (jobs in no
@@ -57,20 +60,28 @@ inline void to_plan(const SlotClasses &c, int njobs, int base, QueuePlan *p,
     }
     p->off.push_back((int)p->order.size());
   }
+  size_t small = 0;
+  for (size_t k = 1; k < c.size.size(); k++)
+    if (c.size[k] < c.size[small]) small = k;
+  for (size_t k = 0; k < c.size.size(); k++)
+    p->fb.push_back(k != small && c.size[k] >= c.size[small] ? (int)small : -1);
 }
 
 // The mixed-slot plan from one run's loop times: job i took ms[i] on g0[i]
 // workgroups and is modelled to take ms[i] sqrt(g0[i] / groups(i, size)) on
 // a slot of `size` (groups(i, size): the workgroups its loop takes there).
-// The mix of an optional slot of 64-128 groups for the longest job, up to
+// The mixes of an optional slot of 64-128 groups for the longest job, up to
 // four big slots (32 to 128 groups) and slots of `base` groups, within
-// `budget` workgroups, with the smallest modelled span is kept if it beats
-// equal base slots by 10 %; one queue per slot size, longest jobs first.
-inline bool sqrt_plan(const std::vector<double> &ms, const std::vector<double> &g0,
-                      const std::function<double(int, int)> &groups, int budget, int base,
-                      SlotClasses *out) {
+// `budget` workgroups, are ranked by modelled span; the `want` best (fewer
+// if fewer beat equal base slots by 10 %) go to out, best first; one queue
+// per slot size, longest jobs first.  The model errs (a loop's serial pick
+// walk does not shrink with more groups), so the caller measures them.
+inline bool sqrt_plans(const std::vector<double> &ms, const std::vector<double> &g0,
+                       const std::function<double(int, int)> &groups, int budget, int base,
+                       int want, std::vector<SlotClasses> *out) {
   const int njobs = (int)ms.size();
-  if (njobs == 0 || (int)g0.size() != njobs || base < 1) return false;
+  out->clear();
+  if (njobs == 0 || (int)g0.size() != njobs || base < 1 || want < 1) return false;
   auto time_on = [&](int i, int size) {
     const double g = groups(i, size);
     return g > 0 ? ms[(size_t)i] * std::sqrt(g0[(size_t)i] / g) : 1e300;
@@ -109,40 +120,53 @@ inline bool sqrt_plan(const std::vector<double> &ms, const std::vector<double> &
   };
   if (budget < base) return false;
   const double m0 = schedule(sizes_of(0, 0, base), nullptr);
-  std::vector<int> best_sizes;
-  double best_m = m0;
+  std::vector<std::pair<double, std::vector<int>>> ranked;
   for (int s1 : {0, 64, 96, 128})
     for (int sb : {32, 48, 64, 96, 128})
       for (int nb = 0; nb <= 4; nb++) {
         if (sb <= base || (s1 > 0 ? s1 : 0) + nb * sb > budget - base) continue;
         if (s1 == 0 && nb == 0) continue;
-        const std::vector<int> sz = sizes_of(s1, nb, sb);
+        std::vector<int> sz = sizes_of(s1, nb, sb);
         const double m = schedule(sz, nullptr);
-        if (m < best_m) {
-          best_m = m;
-          best_sizes = sz;
-        }
+        if (m <= 0.9 * m0) ranked.push_back({m, std::move(sz)});
       }
-  if (best_sizes.empty() || best_m > 0.9 * m0) return false;
-  std::vector<int> slot_of;
-  schedule(best_sizes, &slot_of);
-  SlotClasses c;
-  for (int sz : best_sizes) {
-    size_t k = 0;
-    while (k < c.size.size() && c.size[k] != sz) k++;
-    if (k == c.size.size()) {
-      c.size.push_back(sz);
-      c.count.push_back(0);
-      c.queue.push_back({});
+  std::stable_sort(ranked.begin(), ranked.end(),
+                   [](const auto &x, const auto &y) { return x.first < y.first; });
+  for (const auto &r : ranked) {
+    if ((int)out->size() == want) break;
+    std::vector<int> slot_of;
+    schedule(r.second, &slot_of);
+    SlotClasses c;
+    for (int sz : r.second) {
+      size_t k = 0;
+      while (k < c.size.size() && c.size[k] != sz) k++;
+      if (k == c.size.size()) {
+        c.size.push_back(sz);
+        c.count.push_back(0);
+        c.queue.push_back({});
+      }
+      c.count[k]++;
     }
-    c.count[k]++;
+    for (int i : by_time) {
+      size_t k = 0;
+      while (c.size[k] != slot_of[(size_t)i]) k++;
+      c.queue[k].push_back(i);
+    }
+    bool dup = false;  // another size mix can schedule to the same plan
+    for (const auto &o : *out)
+      dup |= o.size == c.size && o.count == c.count && o.queue == c.queue;
+    if (!dup) out->push_back(std::move(c));
   }
-  for (int i : by_time) {
-    size_t k = 0;
-    while (c.size[k] != slot_of[(size_t)i]) k++;
-    c.queue[k].push_back(i);
-  }
-  *out = c;
+  return !out->empty();
+}
+
+// The best plan of sqrt_plans.
+inline bool sqrt_plan(const std::vector<double> &ms, const std::vector<double> &g0,
+                      const std::function<double(int, int)> &groups, int budget, int base,
+                      SlotClasses *out) {
+  std::vector<SlotClasses> v;
+  if (!sqrt_plans(ms, g0, groups, budget, base, 1, &v)) return false;
+  *out = v[0];
   return true;
 }
 

@@ -108,6 +108,20 @@ static void check_plan_shape(const siting::SlotClasses &c, int njobs, int budget
     if (!c.queue[k].empty()) CHECK(c.count[k] >= 1);
     for (int i : c.queue[k]) CHECK(cap[(size_t)i] == c.size[k]);
   }
+  // fallback queues: another class with slots no larger, and one class
+  // (the smallest slots) with none
+  CHECK(p.fb.size() == c.size.size());
+  int none = 0;
+  for (size_t k = 0; k < p.fb.size(); k++) {
+    const int f = p.fb[k];
+    if (f < 0) {
+      none++;
+      continue;
+    }
+    CHECK(f < (int)c.size.size() && f != (int)k && c.size[(size_t)f] <= c.size[k]);
+    CHECK(p.fb[(size_t)f] < 0);
+  }
+  CHECK(none >= 1);
 }
 
 static void check_slot_plans() {
@@ -123,11 +137,24 @@ static void check_slot_plans() {
       gmax[(size_t)i] = 4 + (int)(rng() % 125);
     }
     auto groups = [&](int i, int size) { return (double)std::min(size, gmax[(size_t)i]); };
+    // the model's best few plans: each a valid plan, all distinct, the first
+    // the one sqrt_plan gives
+    std::vector<siting::SlotClasses> top;
+    const bool any = siting::sqrt_plans(ms, g0, groups, budget, base, 3, &top);
+    CHECK(any == !top.empty() && top.size() <= 3);
+    for (size_t x = 0; x < top.size(); x++) {
+      check_plan_shape(top[x], njobs, budget, base);
+      for (size_t y = 0; y < x; y++)
+        CHECK(!(top[x].size == top[y].size && top[x].count == top[y].count &&
+                top[x].queue == top[y].queue));
+    }
     siting::SlotClasses c;
     if (!siting::sqrt_plan(ms, g0, groups, budget, base, &c)) {
+      CHECK(!any);
       c = siting::SlotClasses{{base}, {std::min(budget / base, njobs)}, {{}}};
       for (int i = 0; i < njobs; i++) c.queue[0].push_back(i);
     } else {
+      CHECK(any && c.size == top[0].size && c.count == top[0].count && c.queue == top[0].queue);
       made++;
     }
     check_plan_shape(c, njobs, budget, base);
