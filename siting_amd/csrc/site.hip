@@ -1259,14 +1259,20 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           }
           return walk_area(cid, cpos, nearp);
         };
+        // the largest key of the rekeyed set (0 while it is empty), kept
+        // across steps and refreshed only when the set changes
+        uint64_t rmax = 0;
+        auto rmax_of_set = [&]() -> uint64_t {
+          return wave_max_u64_dpp(lane < rn ? r_key[lane] : 0);
+        };
         while (k < kmax) {
           SITE_PROF_ADD(11, 1)
           const uint64_t bv = wave_max_u64_dpp(h.k[0]);
-          if (rn > 0) {
-            const uint64_t rkv = lane < rn ? r_key[lane] : 0;
-            const uint64_t rv = wave_max_u64_dpp(rkv);
+          if (rn > 0 && rmax >= bv) {
             // (equal only to the bound a rekeyed lane left, which is its own)
-            if (rv >= bv) {
+            const uint64_t rv = rmax;
+            const uint64_t rkv = lane < rn ? r_key[lane] : 0;
+            {
               const int e = (int)__builtin_ctzll(__ballot(rkv == rv));
               const uint32_t rpos = r_pos[e];
               const int mark = r_mark[e];
@@ -1290,6 +1296,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
                   r_mark[e] = k;
                 }
                 __builtin_amdgcn_wave_barrier();
+                rmax = rmax_of_set();
                 continue;
               }
               if ((rv >> 32) == 0) {  // no candidate adds area
@@ -1312,7 +1319,9 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
                 r_pos[e] = r_pos[rn - 1];
                 r_mark[e] = r_mark[rn - 1];
               }
+              __builtin_amdgcn_wave_barrier();
               rn--;
+              rmax = rmax_of_set();
               k++;
               continue;
             }
@@ -1346,13 +1355,15 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
             }
             nevals++;
             const int area = rekey_area(~(uint32_t)bv, pos, nearp);
+            const uint64_t rk = ((uint64_t)(uint32_t)area << 32) | (uint32_t)bv;
             if (lane == 0) {
-              r_key[rn] = ((uint64_t)(uint32_t)area << 32) | (uint32_t)bv;
+              r_key[rn] = rk;
               r_pos[rn] = pos;
               r_mark[rn] = k;
             }
             __builtin_amdgcn_wave_barrier();
             rn++;
+            rmax = rk > rmax ? rk : rmax;
             if (lane == src) lane_pop(h);
             continue;
           }
