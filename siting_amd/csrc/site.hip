@@ -788,7 +788,9 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   __shared__ uint64_t xo[kPickWaves - 1][kWave];  // their largest keys not kept
 
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wid = tid >> 6;
+  // (the wavefront index by readfirstlane: wave-uniform, so that branches
+  // on it are scalar branches)
+  const int lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int n = a.n, roi = a.roi, nr = a.nr, nwpr = a.nwpr, cw = a.cw;
   const int Q = a.Q, upw = a.upw, kpu = a.kpu, nunits = a.nunits;
   const int r2 = (2 * roi) * (2 * roi);
@@ -1085,11 +1087,65 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       }
       __syncthreads();
       SITE_PROF_MARK(7)
+      // The walk's order, sorted once: wavefront 0's lanes hold 256 entries
+      // (each lane's three keys and its largest dropped key, a bound entry),
+      // and the order in which the walk would take the lanes' largest keys
+      // one by one is their descending order.  Every thread counts, for one
+      // entry, the larger entries among a quarter of them (keys are distinct,
+      // 0 an empty slot); the 64 largest go to sk/sp by rank, and the walk
+      // steps through them with readlanes.  (The rows of xk/xp are free once
+      // wavefront 0 has merged them.)
+      static_assert(3 * (kPickWaves - 1) >= 9 && kLoopThreads == 16 * kWave,
+                    "the sort's rows of xk/xp and its four quarters of 256 entries");
+      uint64_t *ek = &xk[0][0];         // entry s * kWave + l: lane l's s-th
+      uint32_t *ep = &xp[0][0];
+      int *erank = (int *)&xp[4][0];    // larger entries than entry e
+      uint64_t *sk = &xk[4][0];         // the 64 largest, descending (0: none)
+      uint32_t *sp = &xp[8][0];
       if (wid == 0) {
 #pragma unroll
         for (int s = 0; s < 3 * (kPickWaves - 1); s++) top3_push(h, xk[s][lane], xp[s][lane]);
 #pragma unroll
         for (int s = 0; s < kPickWaves - 1; s++) h.over = xo[s][lane] > h.over ? xo[s][lane] : h.over;
+        // (each lane reads its own slots above before writing them here)
+        ek[lane] = h.k[0];
+        ek[kWave + lane] = h.k[1];
+        ek[2 * kWave + lane] = h.k[2];
+        ek[3 * kWave + lane] = h.over;
+        ep[lane] = h.p[0];
+        ep[kWave + lane] = h.p[1];
+        ep[2 * kWave + lane] = h.p[2];
+        ep[3 * kWave + lane] = kGhost;
+#pragma unroll
+        for (int s = 0; s < 4; s++) erank[s * kWave + lane] = 0;
+        sk[lane] = 0;
+      }
+      __syncthreads();
+      {
+        const int e = tid & (4 * kWave - 1), q = tid / (4 * kWave);
+        const uint64_t mine = ek[e];
+        int c = 0;
+#pragma unroll 16
+        for (int j = 0; j < kWave; j++) c += ek[q * kWave + j] > mine ? 1 : 0;
+        if (mine != 0 && c != 0) atomicAdd(&erank[e], c);
+      }
+      __syncthreads();
+      if (wid == 0) {
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+          const int e = s * kWave + lane, r = erank[e];
+          const uint64_t kv = ek[e];
+          if (kv != 0 && r < kWave) {
+            sk[r] = kv;
+            sp[r] = ep[e];
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const uint64_t skr = sk[lane];
+        const uint32_t spr = sp[lane];
+        int si = 0;  // the next entry of the order
         // Lane j < k holds pick j.
         uint64_t pk = 0;
         uint32_t pp = 0;
@@ -1148,8 +1204,9 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
             while (((acc = __hip_atomic_load(&mb_acc, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) >> 40) <
                    (unsigned long long)(kLoopWaves - 1))
               __builtin_amdgcn_s_sleep(1);
-          acc = __shfl(acc, 0, kWave);
-          return own + (int)(acc & ((1ull << 40) - 1));
+          // (lane 0's sum by readlane: a wave-uniform value, so that the
+          // walk's control flow that depends on it stays scalar)
+          return own + __builtin_amdgcn_readlane((int)(uint32_t)(acc & ((1ull << 40) - 1)), 0);
         };
         auto walk_area = [&](uint32_t cid, uint32_t cpos, uint64_t picks) -> int {
           SITE_PROF(const unsigned long long t0 = wall_clock64();)
@@ -1280,7 +1337,12 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         };
         while (k < kmax) {
           SITE_PROF_ADD(11, 1)
-          const uint64_t bv = wave_max_u64_dpp(h.k[0]);
+          // the largest key not yet taken; past the 64th, the 64th stands
+          // for the rest as a bound entry
+          const int sj = si < kWave ? si : kWave - 1;
+          const uint64_t bv =
+              ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(skr >> 32), sj) << 32) |
+              (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)skr, sj);
           if (rn > 0 && rmax >= bv) {
             // (equal only to the bound a rekeyed lane left, which is its own)
             const uint64_t rv = rmax;
@@ -1303,7 +1365,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
                   break;
                 }
                 nevals++;
-                const int area = rekey_area(~(uint32_t)rv, rpos, nearp);
+                const int area = __builtin_amdgcn_readfirstlane(rekey_area(~(uint32_t)rv, rpos, nearp));
                 if (lane == 0) {
                   r_key[e] = ((uint64_t)(uint32_t)area << 32) | (uint32_t)rv;
                   r_mark[e] = k;
@@ -1344,9 +1406,8 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
             break;
           }
           if (k == 0) head = (int)(bv >> 32);
-          // the lane that holds it (keys are distinct)
-          const int src = (int)__builtin_ctzll(__ballot(h.k[0] == bv));
-          const uint32_t pos = (uint32_t)__builtin_amdgcn_readlane((int)h.p[0], src);
+          const uint32_t pos =
+              si < kWave ? (uint32_t)__builtin_amdgcn_readlane((int)spr, sj) : kGhost;
           // a bound entry (keys the lane does not hold may come next), or a
           // stale key (intervis 0: its area is only a bound)
           if (pos >> 31) {
@@ -1367,7 +1428,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
               break;
             }
             nevals++;
-            const int area = rekey_area(~(uint32_t)bv, pos, nearp);
+            const int area = __builtin_amdgcn_readfirstlane(rekey_area(~(uint32_t)bv, pos, nearp));
             const uint64_t rk = ((uint64_t)(uint32_t)area << 32) | (uint32_t)bv;
             if (lane == 0) {
               r_key[rn] = rk;
@@ -1377,7 +1438,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
             __builtin_amdgcn_wave_barrier();
             rn++;
             rmax = rk > rmax ? rk : rmax;
-            if (lane == src) lane_pop(h);
+            si++;
             continue;
           }
           if (ivm && k > 0) {
@@ -1391,7 +1452,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
             pk = bv;
             pp = pos;
           }
-          if (lane == src) lane_pop(h);
+          si++;
           k++;
         }
         SITE_PROF_ADD(12, k)
