@@ -82,7 +82,8 @@ namespace siting {
 //   6 end-of-round barrier, 7 key gather, 8 loop workgroups,
 //   9 exact computations, 10 their time, 11 walk steps, 12 picks,
 //   13-19 walk ends: computation budget, rekeyed set full, bound entry,
-//   stale key, no area left, pick cap, round without a pick.
+//   stale key, no area left, pick cap, round without a pick;
+//   20 the merge and sort of the walk's entries (the walk, 0, follows it).
 #ifdef SITING_PHASE_PROFILE
 constexpr int kProfWords = 32;
 __device__ unsigned long long g_site_prof[kProfWords];
@@ -1146,6 +1147,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         const uint64_t skr = sk[lane];
         const uint32_t spr = sp[lane];
         int si = 0;  // the next entry of the order
+        SITE_PROF_MARK(20)
         // Lane j < k holds pick j.
         uint64_t pk = 0;
         uint32_t pp = 0;

@@ -8,7 +8,8 @@ per job, the loop's time per round and phase, averaged over the loop's
 workgroups (thread 0 of each, 100 MHz device clock):
 
   gather    every workgroup reads the published unit keys
-  walk      the pick walk down the global key order (one wavefront; its
+  sort      the walk's 256 entries merged into one wavefront and sorted
+  walk      the pick walk down the sorted entries (one wavefront; its
             exact area computations are shown separately)
   book      counters, pick hashes and, when the walk read cum, the walk
             barrier
@@ -44,6 +45,7 @@ import siting_amd as sa
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PHASES = ("walk", "book", "union", "update", "reeval", "publish", "endbar", "gather")
+SORT = 20  # the counter of the walk's sort
 ENDS = ("budget", "rekeyed_full", "bound_entry", "stale_key", "no_area", "pick_cap",
         "no_pick")
 NWORDS = 32
@@ -85,6 +87,7 @@ def profile(job: sa.Job, lib) -> dict:
     rounds = max(1, st.rounds)
     per = groups * rounds
     us = {nm: round(float(buf[q]) * 0.01 / per, 2) for q, nm in enumerate(PHASES)}
+    us["sort"] = round(float(buf[SORT]) * 0.01 / per, 2)
     evals = float(buf[9]) / per
     steps = float(buf[11]) / per
     walk_eval_us = float(buf[10]) * 0.01 / per
@@ -159,7 +162,7 @@ def main(argv=None):
                     print(f"{name:>6} n={w['nrows']} roi={w['roi']} G={r['groups']} "
                           f"SITE {r['site_ms']} ms, {r['rounds']} rounds of "
                           f"{r['round_us']} us ({r['picks_per_round']} picks): "
-                          + " ".join(f"{k} {u[k]}" for k in PHASES)
+                          + " ".join(f"{k} {u[k]}" for k in PHASES + ("sort",))
                           + f" | walk: {r['walk_computations_per_round']} computations x "
                           f"{r['walk_computation_us']} us, {r['walk_steps_per_round']} steps x "
                           f"{r['walk_step_us']} us | ends {r['walk_ends']}", flush=True)
