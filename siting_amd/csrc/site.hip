@@ -2233,13 +2233,16 @@ siting_status site_make_args(const Geometry &g, const int32_t *obs,
     const char *w = std::getenv("SITING_SITE_WALK_EVALS");
     if (w && *w) ev = std::max(0, std::min(kMaxBatch, std::atoi(w)));
     a.walk_evals = ev;
-    // rekeys per round: 8 (intervis 0, measured on roi 25-150 jobs of 894^2
-    // to 4096^2: 10-33 % less SITE time, 8 and 16 alike, 2 to 4 behind);
-    // under intervis only where the walk's exact computations pay, by the
-    // same choice (the 2048^2 roi-80 job: 2867 -> 1560 rounds, SITE 137 ->
-    // 113 ms; a 7857^2 roi-50 request, where they are off: 2 % slower with
-    // them).  SITING_SITE_NEAR_EVALS overrides.
-    int ne = g.p.intervis ? (a.iv_multi && ev > 0 ? 8 : 0) : 8;
+    // rekeys per round: 16 (intervis 0, measured on roi 25-150 jobs of 894^2
+    // to 4096^2: 10-33 % less SITE time than none, 2 to 4 behind; with the
+    // computations' pick groups matched to their pick count, 16 against 8:
+    // the uneven batch's SITE launch 65.1 -> 64.5 ms, a 2897^2 roi-100 job
+    // 35.7 -> 33.7 ms, the 4096^2 job flat); under intervis only where the
+    // walk's exact computations pay, by the same choice (the 2048^2 roi-80
+    // job: 2867 -> 1560 rounds, SITE 137 -> 113 ms with 8; 91.0 -> 88.7 ms
+    // with 16 over 8; a 7857^2 roi-50 request, where they are off: 2 %
+    // slower with them).  SITING_SITE_NEAR_EVALS overrides.
+    int ne = g.p.intervis ? (a.iv_multi && ev > 0 ? 16 : 0) : 16;
     const char *ne_env = std::getenv("SITING_SITE_NEAR_EVALS");
     if (ne_env && *ne_env) ne = std::max(0, std::min(kMaxBatch, std::atoi(ne_env)));
     a.near_evals = g.p.intervis && !a.iv_multi ? 0 : ne;
