@@ -1044,10 +1044,12 @@ siting_status siting_batch_create(int device, int njobs, const siting_params *pa
     j->site_state = true;  // stats read the SITE state
     // the tuning cache's SITE choices are measured for a job's loop alone on
     // up to half the CUs; in work-queue slots of 4-128 groups beside 60
-    // other loops the built-in rules measured faster (uneven 64-job batch:
-    // SITE launch 96 ms with the rules, 116 with the cached choices)
+    // other loops the built-in column rule measured faster (uneven 64-job
+    // batch: SITE launch 96 ms with the rules, 116 with the cached choices),
+    // and a stale threshold of head - head/16 (the uneven batch's SITE
+    // launch with shifts 2..6: 81.1, 65.5, 61.7, 65.1, 77.8 ms)
     j->g.t.site_q = 0;
-    j->g.t.site_tau_shift = 0;
+    j->g.t.site_tau_shift = 4;
     b->jobs.push_back(j);
   }
   SITING_HIP(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device));

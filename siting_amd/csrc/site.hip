@@ -2014,11 +2014,13 @@ int choose_q(int n, int roi, int nsheds, int tuned = 0) {
 
 // Update candidates whose area is below tau = head - head / 2^s (head: the
 // round's largest key's area) are not decremented but kept as stale bounds
-// (s = 5; a job's tuning may choose 1..20; SITING_SITE_TAU_SHIFT overrides,
-// 0 turning it off).
+// (s = 3: over the uneven 64-job batch's jobs one by one, SITE summed 502,
+// 480, 493 and 494 ms with s = 2, 3, 4 and 5, and the 1000^2 roi-50 job
+// took 1.65, 1.69, 2.24 and 2.24 ms; a job's tuning may choose 1..20;
+// SITING_SITE_TAU_SHIFT overrides, 0 turning it off).
 int tau_shift(int tuned = 0) {
   const char *e = std::getenv("SITING_SITE_TAU_SHIFT");
-  const int t = e && *e ? std::atoi(e) : (tuned >= 1 && tuned <= 20 ? tuned : 5);
+  const int t = e && *e ? std::atoi(e) : (tuned >= 1 && tuned <= 20 ? tuned : 3);
   return t < 0 ? 0 : (t > 20 ? 20 : t);
 }
 

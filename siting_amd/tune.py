@@ -12,9 +12,12 @@ own knobs, the others held at the rules:
   VIX       LDS halo 8, 16, 32
   VIEWSHED  threads per observer 128, 256, 512, times bitset mode (two LDS
             bitsets, one LDS bitset, HBM; the modes whose bitsets fit)
-  SITE      column classes 2, 3, times stale threshold shift 4, 5, 6;
-            and, on the same terrain with intervis 1, the pick walk's exact
-            area computations per round: none, 2, 8 or 16
+  SITE      column classes 2, 3, times stale threshold shift 2 to 6;
+            and, on the same terrain with intervis 1, column classes 2, 3,
+            times the pick walk's exact area computations per round: none,
+            2, 8 or 16.  The column count kept is the one whose best
+            intervis-0 and intervis-1 times sum least (both kinds of job
+            share it), with its own best shift and computations
 
 A choice replaces the rule's only when it is at least 3 % faster.  The job
 is then run again with the chosen values, and its vix, observers, viewshed
@@ -91,7 +94,7 @@ def rule_choice(roi: int, ntests: int, n: int, nsheds: int) -> dict:
     q_list = 3.14159265358979 * 4.0 * roi * roi * nsheds / (n * n)
     return {"vix_halo": 32 if roi >= 64 and ntests >= 64 else 16,
             "viewshed_threads": 256, "viewshed_mode": modes[0],
-            "site_q": 3 if q_list >= 3000.0 else 2, "site_tau_shift": 5,
+            "site_q": 3 if q_list >= 3000.0 else 2, "site_tau_shift": 3,
             "site_walk_evals": 17 if roi <= 40 else 9 if roi > 75 else 1}
 
 
@@ -132,25 +135,19 @@ def tune_class(key: str, roi: int, ntests: int, edge: int, reps: int, log):
         th, m = _pick(tw, (rule["viewshed_threads"], rule["viewshed_mode"]))
         choice["viewshed_threads"], choice["viewshed_mode"] = th, m
         times["viewshed"] = tw[(th, m)]
-        # SITE
+        # SITE: each column count's best stale threshold shift
         ts = {}
         for q in (2, 3):
-            for s in (4, 5, 6):
+            for s in (2, 3, 4, 5, 6):
                 job.set_tuning(sa.Tuning(**dict(rule, site_q=q, site_tau_shift=s)))
                 ts[(q, s)] = _time_stage(job, "site", reps)
-        q, s = _pick(ts, (rule["site_q"], rule["site_tau_shift"]))
-        choice["site_q"], choice["site_tau_shift"] = q, s
-        times["site"] = ts[(q, s)]
+        best_s = {q: _pick({s: t for (qq, s), t in ts.items() if qq == q},
+                           rule["site_tau_shift"]) for q in (2, 3)}
         rule_ms = {"vix": tv[rule["vix_halo"]],
                    "viewshed": tw[(rule["viewshed_threads"], rule["viewshed_mode"])],
                    "site": ts[(rule["site_q"], rule["site_tau_shift"])]}
-        # the chosen values must give the rules' bits
-        job.set_tuning(sa.Tuning(**choice))
-        job.run()
-        job.sync()
-        if _outputs(job) != ref:
-            raise SystemExit(f"tuned run differs from the rules' for {w}")
-    # intervis: the pick walk's exact computations, on the same terrain
+    # intervis, on the same terrain: each column count's best number of the
+    # pick walk's exact computations
     pi = sa.make_params(n, roi, 100, ntests, 16, n * n // 256, 1)
     with sa.Job(pi) as job:
         job.upload_dem(dem)
@@ -159,17 +156,37 @@ def tune_class(key: str, roi: int, ntests: int, edge: int, reps: int, log):
         job.sync()
         ref_iv = _outputs(job)
         te = {}
-        for ev in (1, 3, 9, 17):
-            job.set_tuning(sa.Tuning(**dict(choice, site_walk_evals=ev)))
-            te[ev] = _time_stage(job, "site", max(1, reps - 1))
-        choice["site_walk_evals"] = _pick(te, rule["site_walk_evals"])
-        times["site_iv"] = te[choice["site_walk_evals"]]
-        rule_ms["site_iv"] = te[rule["site_walk_evals"]]
+        for q in (2, 3):
+            for ev in (1, 3, 9, 17):
+                job.set_tuning(sa.Tuning(**dict(choice, site_q=q, site_walk_evals=ev)))
+                te[(q, ev)] = _time_stage(job, "site", max(1, reps - 1))
+        best_ev = {q: _pick({ev: t for (qq, ev), t in te.items() if qq == q},
+                            rule["site_walk_evals"]) for q in (2, 3)}
+        # one column count serves both kinds of job: the one whose intervis-0
+        # and intervis-1 loops take least time together (measured on the
+        # 2048^2 roi-80 intervis job: 88.9 ms with 3, 100.8 with 2, where the
+        # intervis-0 loop alone had chosen 2)
+        both = {q: ts[(q, best_s[q])] + te[(q, best_ev[q])] for q in (2, 3)}
+        q = _pick(both, rule["site_q"])
+        choice["site_q"] = q
+        choice["site_tau_shift"] = best_s[q]
+        choice["site_walk_evals"] = best_ev[q]
+        times["site"] = ts[(q, best_s[q])]
+        times["site_iv"] = te[(q, best_ev[q])]
+        rule_ms["site_iv"] = te[(rule["site_q"], rule["site_walk_evals"])]
         job.set_tuning(sa.Tuning(**choice))
         job.site()
         job.sync()
         if _outputs(job) != ref_iv:
             raise SystemExit(f"tuned intervis run differs from the rules' for {w}")
+    # the chosen values must give the rules' bits
+    with sa.Job(p) as job:
+        job.upload_dem(dem)
+        job.set_tuning(sa.Tuning(**choice))
+        job.run()
+        job.sync()
+        if _outputs(job) != ref:
+            raise SystemExit(f"tuned run differs from the rules' for {w}")
     log(f"{key} roi {roi} targets {ntests} edge {edge}: {choice} "
         f"ms {times} (rules {rule_ms})")
     return choice, times, rule_ms
