@@ -185,6 +185,8 @@ def use_profiling_build() -> ctypes.CDLL:
     handle = lib()
     handle.siting_phase_profile_read.restype = _I
     handle.siting_phase_profile_read.argtypes = [_VP, ctypes.c_int, ctypes.c_int]
+    handle.siting_phase_trace_read.restype = _I
+    handle.siting_phase_trace_read.argtypes = [_VP, ctypes.c_size_t]
     return handle
 
 

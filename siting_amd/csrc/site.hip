@@ -84,9 +84,17 @@ namespace siting {
 //   13-19 walk ends: computation budget, rekeyed set full, bound entry,
 //   stale key, no area left, pick cap, round without a pick;
 //   20 the merge and sort of the walk's entries (the walk, 0, follows it).
+// The profiling build also traces the first kTraceRounds rounds of a loop
+// of up to kTraceGroups workgroups: thread 0 of each workgroup stores the
+// device clock at the round's start, after the key gather, after the walk,
+// after the publish, after the bookkeeping, the union, the update lists and
+// the re-evaluation (g_site_trace[mark][round][group], marks 0-7 in that
+// order), which shows how far the workgroups drift apart within a round.
 #ifdef SITING_PHASE_PROFILE
 constexpr int kProfWords = 32;
+constexpr int kTraceMarks = 8, kTraceRounds = 256, kTraceGroups = 128;
 __device__ unsigned long long g_site_prof[kProfWords];
+__device__ unsigned long long g_site_trace[kTraceMarks][kTraceRounds][kTraceGroups];
 #define SITE_PROF(stmt) stmt
 #else
 #define SITE_PROF(stmt)
@@ -752,7 +760,11 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   }
 #define SITE_PROF_ADD(k, v) \
   if ((threadIdx.x & 63) == 0) pf_acc[k] += (unsigned long long)(v);
+#define SITE_TRACE(m)                                              \
+  if (threadIdx.x == 0 && pf_r < kTraceRounds && g < kTraceGroups) \
+    g_site_trace[m][pf_r][g] = wall_clock64();
 #else
+#define SITE_TRACE(m)
 #define SITE_PROF_MARK(k)
 #define SITE_PROF_ADD(k, v)
 #endif
@@ -1018,8 +1030,11 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     for (int q = 0; q < kProfWords; q++) pf_acc[q] = 0;
     pf_last = wall_clock64();
   })
+  SITE_PROF(int pf_r = 0;)
   while (started) {
     SITE_PROF_MARK(6)
+    SITE_PROF(if (tid == 0) pf_r = ph_round;)
+    SITE_TRACE(0)
     if (tid == 0) mb_seq = 0;
     if (wid == kLoopWaves - 1) {
       bool bad = false;
@@ -1088,6 +1103,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       }
       __syncthreads();
       SITE_PROF_MARK(7)
+      SITE_TRACE(1)
       // The walk's order, sorted once: wavefront 0's lanes hold 256 entries
       // (each lane's three keys and its largest dropped key, a bound entry),
       // and the order in which the walk would take the lanes' largest keys
@@ -1496,6 +1512,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       break;
     }
     SITE_PROF_MARK(0)
+    SITE_TRACE(2)
     int k = bw_n;
     // no pick while the largest key is stale: the round only re-evaluates
     const bool eval_only = k == 0 && bw_head > 0;
@@ -1592,6 +1609,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     }
 
     SITE_PROF_MARK(1)
+    SITE_TRACE(4)
     // ---- union of the picks into cum (cpp:393-419)
     Winner W;
     {
@@ -1680,6 +1698,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     // windows of kMaxList items; a candidate in several ranges is listed
     // once (seen bitmap, cleared at the end of the round).
     SITE_PROF_MARK(2)
+    SITE_TRACE(5)
     if (tid == 0) rng_n = 0;
     __syncthreads();
     for (int t = tid; t < kc * upw; t += kLoopThreads) {
@@ -1814,6 +1833,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       __syncthreads();
     }
     SITE_PROF_MARK(3)
+    SITE_TRACE(6)
     for (int i = tid; i < (cnt + 31) / 32; i += kLoopThreads) seen[i] = 0;
     if (stop) break;
     // before the re-evaluation reads cum (intervis rounds re-evaluate
@@ -1886,6 +1906,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       if (tid == 0) list_n = 0;
     }
     SITE_PROF_MARK(4)
+    SITE_TRACE(7)
     for (int ul = wid; ul < upw; ul += kLoopWaves)
       if (dirty[ul]) unit_top2(ul);
     __syncthreads();
@@ -1894,6 +1915,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     publish(parity);
     if (ivm) publish_cells(parity);
     SITE_PROF_MARK(5)
+    SITE_TRACE(3)
     if (!grid_sync(&a.st->barrier, bar_target, (unsigned)G, a.spin_limit)) {
       niter = -1;
       break;
@@ -2384,6 +2406,22 @@ extern "C" siting_status siting_phase_profile_read(unsigned long long *out, int 
     unsigned long long z[siting::kProfWords] = {0};
     SITING_HIP(hipMemcpyToSymbol(HIP_SYMBOL(siting::g_site_prof), z, sizeof(z)));
   }
+  return SITING_OK;
+}
+
+// Profiling build only: copies the round trace (kTraceMarks x kTraceRounds
+// x kTraceGroups words, 8 x 256 x 128, in that order; 0 where no
+// workgroup stored) and clears it.
+extern "C" siting_status siting_phase_trace_read(unsigned long long *out, size_t n) {
+  constexpr size_t kWords = (size_t)siting::kTraceMarks * siting::kTraceRounds *
+                            siting::kTraceGroups;
+  if (!out || n != kWords) return SITING_EINVAL;
+  SITING_HIP(hipDeviceSynchronize());
+  SITING_HIP(hipMemcpyFromSymbol(out, HIP_SYMBOL(siting::g_site_trace),
+                                 sizeof(unsigned long long) * kWords));
+  std::vector<unsigned long long> z(kWords, 0);
+  SITING_HIP(hipMemcpyToSymbol(HIP_SYMBOL(siting::g_site_trace), z.data(),
+                               sizeof(unsigned long long) * kWords));
   return SITING_OK;
 }
 #endif

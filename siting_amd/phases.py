@@ -46,6 +46,7 @@ import siting_amd as sa
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PHASES = ("walk", "book", "union", "update", "reeval", "publish", "endbar", "gather")
 SORT = 20  # the counter of the walk's sort
+TRACE_SHAPE = (8, 256, 128)  # marks x rounds x workgroups of the round trace
 ENDS = ("budget", "rekeyed_full", "bound_entry", "stale_key", "no_area", "pick_cap",
         "no_pick")
 NWORDS = 32
@@ -76,12 +77,15 @@ def profile(job: sa.Job, lib) -> dict:
     """Runs the job twice (the first warms up) and returns the second run's
     phase profile."""
     buf = np.zeros(NWORDS, np.uint64)
+    trace = np.zeros(TRACE_SHAPE, np.uint64)
     job.run()
     job.sync()
     sa._check(lib.siting_phase_profile_read(buf.ctypes.data, NWORDS, 1))
+    sa._check(lib.siting_phase_trace_read(trace.ctypes.data, trace.size))
     job.run()
     job.sync()
     sa._check(lib.siting_phase_profile_read(buf.ctypes.data, NWORDS, 1))
+    sa._check(lib.siting_phase_trace_read(trace.ctypes.data, trace.size))
     st = job.stats()
     groups = max(1, int(buf[8]))
     rounds = max(1, st.rounds)
@@ -103,8 +107,55 @@ def profile(job: sa.Job, lib) -> dict:
         "walk_steps_per_round": round(steps, 2),
         "walk_step_us": round((us["walk"] - walk_eval_us) / steps, 3) if steps else 0.0,
         "walk_ends": {nm: round(float(buf[13 + q]) / per, 3) for q, nm in enumerate(ENDS)},
+        "drift_us": drift(trace, min(groups, TRACE_SHAPE[2]), st.rounds),
         "selected_sha256": _sha(job.download_selected()),
     }
+
+
+def drift(trace: np.ndarray, groups: int, rounds: int) -> dict:
+    """How far the workgroups drift apart within a round, from the round
+    trace (device clock at each round's start, key gather end, walk end,
+    publish end, and the ends of the bookkeeping, union, update lists and
+    re-evaluation, per workgroup), averaged over the traced rounds, in us:
+    the spread (latest minus earliest workgroup) at the first four marks,
+    the walk's longest duration over its mean, the same for the work after
+    the walk (book to publish) and for each phase of it."""
+    n = min(rounds, TRACE_SHAPE[1])
+    t = trace[:, :n, :groups].astype(np.float64) * 0.01
+    ok = (trace[:, :n, :groups] != 0).all(axis=(0, 2))
+    if not ok.any():
+        return {}
+    t = t[:, ok, :]
+    walk = t[2] - t[1]
+    after = t[3] - t[2]
+    spread = t.max(axis=2) - t.min(axis=2)
+    # the phases after the walk: (start mark, end mark)
+    post = {"book": (2, 4), "union": (4, 5), "update": (5, 6), "reeval": (6, 7),
+            "publish": (7, 3)}
+    over = {nm: round(float(((t[b] - t[a]).max(axis=1) - (t[b] - t[a]).mean(axis=1)).mean()), 2)
+            for nm, (a, b) in post.items()}
+    return {
+        "rounds_traced": int(ok.sum()),
+        "start_spread": round(float(spread[0].mean()), 2),
+        "gather_end_spread": round(float(spread[1].mean()), 2),
+        "walk_end_spread": round(float(spread[2].mean()), 2),
+        "publish_end_spread": round(float(spread[3].mean()), 2),
+        "walk_max_over_mean": round(float((walk.max(axis=1) - walk.mean(axis=1)).mean()), 2),
+        "after_walk_max_over_mean": round(float((after.max(axis=1) - after.mean(axis=1)).mean()), 2),
+        "phase_max_over_mean": over,
+        # the workgroups most often slowest in the update lists and the
+        # re-evaluation, with the share of rounds (a share near 1/G: random)
+        "slowest_update_groups": _most_often((t[6] - t[5]).argmax(axis=1)),
+        "slowest_reeval_groups": _most_often((t[7] - t[6]).argmax(axis=1)),
+        "last_at_walk_end_is_last_at_publish": round(float(
+            (t[2].argmax(axis=1) == t[3].argmax(axis=1)).mean()), 2),
+    }
+
+
+def _most_often(idx: np.ndarray, k: int = 3) -> list:
+    vals, counts = np.unique(idx, return_counts=True)
+    order = np.argsort(-counts)[:k]
+    return [[int(vals[o]), round(float(counts[o]) / len(idx), 2)] for o in order]
 
 
 def _sha(a):
@@ -165,7 +216,8 @@ def main(argv=None):
                           + " ".join(f"{k} {u[k]}" for k in PHASES + ("sort",))
                           + f" | walk: {r['walk_computations_per_round']} computations x "
                           f"{r['walk_computation_us']} us, {r['walk_steps_per_round']} steps x "
-                          f"{r['walk_step_us']} us | ends {r['walk_ends']}", flush=True)
+                          f"{r['walk_step_us']} us | ends {r['walk_ends']} | drift {r['drift_us']}",
+                          flush=True)
 
 
 if __name__ == "__main__":
