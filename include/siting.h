@@ -98,7 +98,7 @@ typedef struct {
   int viewshed_mode;     /* VIEWSHED bitsets: 1 two LDS bitsets (row-major and
                             transposed), 2 one LDS bitset, 3 in HBM; a mode
                             whose bitsets do not fit falls back to the rule */
-  int site_q;            /* SITE: column classes per cell row, 2 or 3 */
+  int site_q;            /* SITE: column classes per cell row, 1..8 */
   int site_tau_shift;    /* SITE: update candidates below head - head/2^s
                             become stale bounds, s in 1..20 */
   int site_walk_evals;   /* SITE, intervis: 1 + the exact area computations

@@ -2020,15 +2020,15 @@ struct Plan {
 // Column classes per cell row.  More classes spread a pick's update window
 // over more workgroups but lengthen the selection scan; three pay only when
 // a pick's expected update list pi*(2roi)^2 * nsheds/n^2 reaches 3000
-// (measured).  A job's tuning (2 or 3) replaces the rule; SITING_SITE_Q
-// overrides both.
+// (measured).  A job's tuning (1 to 8; the tuner tries 2, 3 and 4)
+// replaces the rule; SITING_SITE_Q overrides both.
 int choose_q(int n, int roi, int nsheds, int tuned = 0) {
   const char *e = std::getenv("SITING_SITE_Q");
   if (e && *e) {
     const int q = std::atoi(e);
     return q < 1 ? 1 : (q > 8 ? 8 : q);
   }
-  if (tuned == 2 || tuned == 3) return tuned;
+  if (tuned >= 1 && tuned <= 8) return tuned;
   const double list = 3.14159265358979 * 4.0 * (double)roi * roi * (double)nsheds /
                       ((double)n * (double)n);
   return list >= 3000.0 ? 3 : 2;

@@ -12,8 +12,8 @@ own knobs, the others held at the rules:
   VIX       LDS halo 8, 16, 32
   VIEWSHED  threads per observer 128, 256, 512, times bitset mode (two LDS
             bitsets, one LDS bitset, HBM; the modes whose bitsets fit)
-  SITE      column classes 2, 3, times stale threshold shift 2 to 6;
-            and, on the same terrain with intervis 1, column classes 2, 3,
+  SITE      column classes 2, 3, 4, times stale threshold shift 2 to 6;
+            and, on the same terrain with intervis 1, column classes 2 to 4,
             times the pick walk's exact area computations per round: none,
             2, 8 or 16.  The column count kept is the one whose best
             intervis-0 and intervis-1 times sum least (both kinds of job
@@ -41,6 +41,7 @@ import siting_amd as sa
 ROIS = (25, 50, 100, 150, 200)
 NTESTS = (32, 64)
 EDGES = (1024, 2048, 4096)
+QS = (2, 3, 4)  # SITE column classes tried
 DEFAULT_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune", "tuning.txt")
 
 
@@ -137,12 +138,12 @@ def tune_class(key: str, roi: int, ntests: int, edge: int, reps: int, log):
         times["viewshed"] = tw[(th, m)]
         # SITE: each column count's best stale threshold shift
         ts = {}
-        for q in (2, 3):
+        for q in QS:
             for s in (2, 3, 4, 5, 6):
                 job.set_tuning(sa.Tuning(**dict(rule, site_q=q, site_tau_shift=s)))
                 ts[(q, s)] = _time_stage(job, "site", reps)
         best_s = {q: _pick({s: t for (qq, s), t in ts.items() if qq == q},
-                           rule["site_tau_shift"]) for q in (2, 3)}
+                           rule["site_tau_shift"]) for q in QS}
         rule_ms = {"vix": tv[rule["vix_halo"]],
                    "viewshed": tw[(rule["viewshed_threads"], rule["viewshed_mode"])],
                    "site": ts[(rule["site_q"], rule["site_tau_shift"])]}
@@ -156,17 +157,17 @@ def tune_class(key: str, roi: int, ntests: int, edge: int, reps: int, log):
         job.sync()
         ref_iv = _outputs(job)
         te = {}
-        for q in (2, 3):
+        for q in QS:
             for ev in (1, 3, 9, 17):
                 job.set_tuning(sa.Tuning(**dict(choice, site_q=q, site_walk_evals=ev)))
                 te[(q, ev)] = _time_stage(job, "site", max(1, reps - 1))
         best_ev = {q: _pick({ev: t for (qq, ev), t in te.items() if qq == q},
-                            rule["site_walk_evals"]) for q in (2, 3)}
+                            rule["site_walk_evals"]) for q in QS}
         # one column count serves both kinds of job: the one whose intervis-0
         # and intervis-1 loops take least time together (measured on the
         # 2048^2 roi-80 intervis job: 88.9 ms with 3, 100.8 with 2, where the
         # intervis-0 loop alone had chosen 2)
-        both = {q: ts[(q, best_s[q])] + te[(q, best_ev[q])] for q in (2, 3)}
+        both = {q: ts[(q, best_s[q])] + te[(q, best_ev[q])] for q in QS}
         q = _pick(both, rule["site_q"])
         choice["site_q"] = q
         choice["site_tau_shift"] = best_s[q]

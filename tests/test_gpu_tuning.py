@@ -19,7 +19,7 @@ CASES = [(300, 40, 64, 10, 900, 0, 0.8, 91),
 CHOICES = [dict(vix_halo=8), dict(vix_halo=16), dict(vix_halo=32),
            dict(viewshed_threads=128), dict(viewshed_threads=512),
            dict(viewshed_mode=1), dict(viewshed_mode=2), dict(viewshed_mode=3),
-           dict(site_q=2), dict(site_q=3), dict(site_tau_shift=1),
+           dict(site_q=2), dict(site_q=3), dict(site_q=4), dict(site_tau_shift=1),
            dict(site_tau_shift=9), dict(site_walk_evals=1), dict(site_walk_evals=3),
            dict(site_walk_evals=33),
            dict(vix_halo=32, viewshed_threads=128, viewshed_mode=2, site_q=3,
