@@ -79,7 +79,7 @@ def test_committed_cache_parses(sa, monkeypatch):
         ev = int(r[9]) if len(r) == 10 else 0
         assert roi in (25, 50, 100, 150, 200) and nt in (32, 64) and edge in (1024, 2048, 4096)
         assert halo in (8, 16, 32) and th % 64 == 0 and mode in (1, 2, 3)
-        assert q in (2, 3) and 1 <= s <= 20 and 0 <= ev <= 33
+        assert 1 <= q <= 8 and 1 <= s <= 20 and 0 <= ev <= 33
         key = r[0]
         got = sa.tuning_lookup(key, sa.make_params(edge, roi, 100, nt, 16, 1000, 0))
         assert got.as_dict() == {"vix_halo": halo, "viewshed_threads": th,
