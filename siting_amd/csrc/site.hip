@@ -1276,22 +1276,41 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
             }
             ev_n += __popcll(sv);
           };
-          for (int e0 = 0; e0 < cnt_w; e0 += kWave) {
-            const int e = e0 + lane;
-            const bool ok = e < cnt_w;
-            const int r = r0 + e / ncw, cc = c0 + e % ncw;
-            const uint64_t *rec = cb + (ok ? (size_t)r * a.ncr + cc : 0) * 4;
-            const uint64_t w0 = ld_dev(&rec[0]), w1 = ld_dev(&rec[1]);
-            const uint64_t w2 = ld_dev(&rec[2]), w3 = ld_dev(&rec[3]);
-            const uint32_t b1 = ok ? (uint32_t)(w0 >> 32) : 0, b2 = ok ? (uint32_t)(w1 >> 32) : 0;
-            const uint32_t b3 = ok ? (uint32_t)(w2 >> 32) : 0;
-            m = b3 > m ? b3 : m;  // the rest of the cell's masked candidates
-            // the two largest: only if the pick sees its cell can it gain
-            const bool s1 = sees((uint32_t)w0, b1 != 0);
-            const bool s2 = sees((uint32_t)w2, two && b2 != 0);
-            if (!two) m = b2 > m ? b2 : m;
-            list(s1, (uint32_t)w0, (uint32_t)w1, b1);
-            list(s2, (uint32_t)w2, (uint32_t)w3, b2);
+          // two chunks of 64 cells a pass: their records, then the pick's
+          // shed words at their candidates' cells, each in one round trip
+          // (listed in the same order as one chunk at a time)
+          for (int e0 = 0; e0 < cnt_w; e0 += 2 * kWave) {
+            uint64_t w[2][4];
+            bool ok[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+              const int e = e0 + h * kWave + lane;
+              ok[h] = e < cnt_w;
+              const int r = r0 + e / ncw, cc = c0 + e % ncw;
+              const uint64_t *rec = cb + (ok[h] ? (size_t)r * a.ncr + cc : 0) * 4;
+              w[h][0] = ld_dev(&rec[0]);
+              w[h][1] = ld_dev(&rec[1]);
+              w[h][2] = ld_dev(&rec[2]);
+              w[h][3] = ld_dev(&rec[3]);
+            }
+            uint32_t b1[2], b2[2];
+            bool s1[2], s2[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+              b1[h] = ok[h] ? (uint32_t)(w[h][0] >> 32) : 0;
+              b2[h] = ok[h] ? (uint32_t)(w[h][1] >> 32) : 0;
+              const uint32_t b3 = ok[h] ? (uint32_t)(w[h][2] >> 32) : 0;
+              m = b3 > m ? b3 : m;  // the rest of the cell's masked candidates
+              // the two largest: only if the pick sees its cell can it gain
+              s1[h] = sees((uint32_t)w[h][0], b1[h] != 0);
+              s2[h] = sees((uint32_t)w[h][2], two && b2[h] != 0);
+              if (!two) m = b2[h] > m ? b2[h] : m;
+            }
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+              list(s1[h], (uint32_t)w[h][0], (uint32_t)w[h][1], b1[h]);
+              list(s2[h], (uint32_t)w[h][2], (uint32_t)w[h][3], b2[h]);
+            }
           }
           if (ev_n > kMaxBatch) ev_n = kMaxBatch;
           const uint32_t wm = (uint32_t)wave_max_u64_dpp((uint64_t)m);
