@@ -124,6 +124,11 @@ constexpr int kPickWaves = 4;
 constexpr int kMaxDirtyCells = 1024;
 // Candidates the pick walk may re-key in a round (intervis 0).
 constexpr int kMaxRekey = 8;
+// Chunks of 64 cells the intervis walk's window scan reads per round trip
+#ifndef SITING_WIN_CHUNKS
+#define SITING_WIN_CHUNKS 4
+#endif
+constexpr int kWinChunks = SITING_WIN_CHUNKS;
 static_assert(kMaxBatch <= kWave, "lane j of the walk holds pick j");
 // Position of a bound entry in a lane's key list: a key that stands for
 // keys the lane does not hold (coordinates are below 32767, so no real
@@ -1276,14 +1281,14 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
             }
             ev_n += __popcll(sv);
           };
-          // two chunks of 64 cells a pass: their records, then the pick's
-          // shed words at their candidates' cells, each in one round trip
-          // (listed in the same order as one chunk at a time)
-          for (int e0 = 0; e0 < cnt_w; e0 += 2 * kWave) {
-            uint64_t w[2][4];
-            bool ok[2];
+          // kWinChunks chunks of 64 cells a pass: their records, then the
+          // pick's shed words at their candidates' cells, each in one round
+          // trip (listed in the same order as one chunk at a time)
+          for (int e0 = 0; e0 < cnt_w; e0 += kWinChunks * kWave) {
+            uint64_t w[kWinChunks][4];
+            bool ok[kWinChunks];
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
+            for (int h = 0; h < kWinChunks; h++) {
               const int e = e0 + h * kWave + lane;
               ok[h] = e < cnt_w;
               const int r = r0 + e / ncw, cc = c0 + e % ncw;
@@ -1293,10 +1298,10 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
               w[h][2] = ld_dev(&rec[2]);
               w[h][3] = ld_dev(&rec[3]);
             }
-            uint32_t b1[2], b2[2];
-            bool s1[2], s2[2];
+            uint32_t b1[kWinChunks], b2[kWinChunks];
+            bool s1[kWinChunks], s2[kWinChunks];
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
+            for (int h = 0; h < kWinChunks; h++) {
               b1[h] = ok[h] ? (uint32_t)(w[h][0] >> 32) : 0;
               b2[h] = ok[h] ? (uint32_t)(w[h][1] >> 32) : 0;
               const uint32_t b3 = ok[h] ? (uint32_t)(w[h][2] >> 32) : 0;
@@ -1307,7 +1312,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
               if (!two) m = b2[h] > m ? b2[h] : m;
             }
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
+            for (int h = 0; h < kWinChunks; h++) {
               list(s1[h], (uint32_t)w[h][0], (uint32_t)w[h][1], b1[h]);
               list(s2[h], (uint32_t)w[h][2], (uint32_t)w[h][3], b2[h]);
             }
