@@ -263,6 +263,8 @@ struct SiteArgs {
   int iv_multi;    // intervis: rounds take several picks under cell bounds
   int walk_evals;  // intervis: exact area computations per round's walk
   int near_evals;  // intervis 0: exact area computations per round's walk
+  int full_at;     // intervis 0: update candidates near this many picks of
+                   // a round are counted in full instead of decremented
   int o_b, o_dcur, o_dprev, o_dbits;  // intervis LDS: bounds, dirty cells
 };
 

@@ -1830,10 +1830,28 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           const int cx = (int)(ps >> 16), cy = (int)(ps & 0xffffu);
           const uint64_t *sh = a.sheds + (size_t)C[i] * S;
           int s;
-          if (kIv)
-            s = wave_new_bits(sh, cx, cy, a, W, lane, !ivk);
-          else  // exact area minus the bits this round's picks covered
+          // The exact area: a count of the candidate's box against cum
+          // once the union is complete (intervis rounds of one pick: against
+          // cum | the pick's bits, while the union is written).  Its area
+          // minus the bits this round's picks newly covered is the same
+          // number, but reads each near pick's new bits and then the shed
+          // words they changed, two dependent round trips a pick: slower
+          // even for candidates near one pick (the uneven batch's SITE, one
+          // job at a time, 480 ms with decrements, 461 with them only near
+          // one pick, 443 with counts; the 4096^2 job 4.53 -> 4.25 ms), and
+          // under intervis too (82.2 -> 84.9-91.6 ms), where a candidate
+          // unmasked when last computed also holds its exact new bits.
+          // SITING_SITE_FULL_AT=m decrements intervis-0 candidates near
+          // fewer than m picks.
+          const int mp = list_acc[e];
+          if (!kIv && mp < a.full_at)
             s = T[i] - wave_lost_bits(sh, cx, cy, a, k, bw_pos, lane);
+          else if (kIv && !ivk)
+            s = wave_new_bits(sh, cx, cy, a, W, lane, true);
+          else
+            s = wave_sum_i32(eval_rows_fit(cy, roi, nwpr)
+                                 ? lane_new_bits_rows<kEvalRows>(sh, cx, cy, a, lane, kWave)
+                                 : lane_new_bits<kEvalBatch>(sh, cx, cy, a, W, lane, kWave, false));
           if (lane == 0) list_acc[e] = s;
         }
         __syncthreads();
@@ -2294,6 +2312,14 @@ siting_status site_make_args(const Geometry &g, const int32_t *obs,
     const char *ne_env = std::getenv("SITING_SITE_NEAR_EVALS");
     if (ne_env && *ne_env) ne = std::max(0, std::min(kMaxBatch, std::atoi(ne_env)));
     a.near_evals = g.p.intervis && !a.iv_multi ? 0 : ne;
+    // intervis-0 update candidates within 2*roi of this many of a round's
+    // picks are counted in full against cum, those near fewer decremented
+    // by the bits each pick took: counts throughout (SITING_SITE_FULL_AT
+    // overrides)
+    int fa = 1;
+    const char *fa_env = std::getenv("SITING_SITE_FULL_AT");
+    if (fa_env && *fa_env) fa = std::max(1, std::atoi(fa_env));
+    a.full_at = fa;
   }
   a.n = n;
   a.roi = roi;
