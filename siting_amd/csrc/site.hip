@@ -1691,13 +1691,16 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         if (x >= 0 && x < n && j >= 0 && j < cw) {
           const uint64_t *sw = a.sheds + (size_t)(int)(~(uint32_t)bw_key[i]) * S;
           const uint64_t v = winner_word(sw, nwpr, fbw, dr, dk);
-          if (v) {
+          if (v && a.full_at > 1) {
             const uint64_t old = atomicOr((unsigned long long *)&a.cum[cum_index(x, j, cw)],
                                           (unsigned long long)v);
             nbv = v & ~old;
+          } else if (v) {
+            atomicOr((unsigned long long *)&a.cum[cum_index(x, j, cw)], (unsigned long long)v);
           }
         }
-        st_dev(&a.nb[t], nbv);
+        // the new bits are read only by decrements (full_at > 1)
+        if (a.full_at > 1) st_dev(&a.nb[t], nbv);
       }
       // the update lists are built before the wait; nb and cum are read
       // after it
