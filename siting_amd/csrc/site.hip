@@ -1768,6 +1768,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
     }
     __syncthreads();
     const int total = nrng > 0 ? rng_end[nrng - 1] : 0;
+    bool reeval_rest = true;  // stale candidates left for the re-evaluation below
     for (int w0 = 0; w0 < total; w0 += kMaxList) {
       const int item = w0 + tid;
       if (item < total) {  // cpp:540-544, once per list
@@ -1821,10 +1822,68 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
         my_u += v < 0 ? -v : v;
       }
       __syncthreads();  // counts read before the areas overwrite them
+      // Intervis 0, the first window: the stale candidates whose bound
+      // reaches tau join the window's list and all are counted in one pass
+      // (a group of wavefronts an item when there are few), instead of the
+      // update counts and then the re-evaluation.  Counts are exact against
+      // cum after the union, so a later window that lists a re-evaluated
+      // candidate again counts it again, to the same value.
+      const bool merged = !kIv && hybrid && !stop && w0 == 0 && a.full_at <= 1;
+      if (merged) {
+        for (int i = tid; i < cnt; i += kLoopThreads)
+          if ((U[i] & 3) == 2 && T[i] >= tau) {
+            const int at = atomicAdd(&list_n, 1);
+            if (at < kMaxList) list_i[at] = i;
+          }
+        __syncthreads();
+        const int want = list_n;
+        const int nall = want < kMaxList ? want : kMaxList;
+        reeval_rest = want > kMaxList;
+        if (nall > 0) union_wait();
+        if (nall > 0 && !failed) {
+          for (int e = tid; e < nall; e += kLoopThreads) list_acc[e] = 0;
+          __syncthreads();
+          int gw = 1;
+          while (2 * gw * nall <= kLoopWaves) gw *= 2;
+          const int ngroups = kLoopWaves / gw;
+          const int grp = wid / gw, gl = (wid - grp * gw) * kWave + lane;
+          for (int e0 = grp; e0 < nall; e0 += kWave * ngroups) {
+            const int em = e0 + lane * ngroups;
+            uint32_t mps = 0, mid = 0;
+            if (em < nall) {
+              const int i = list_i[em];
+              mps = P[i];
+              mid = (uint32_t)C[i];
+            }
+            const int ncand = min(kWave, (nall - e0 + ngroups - 1) / ngroups);
+            for (int j = 0; j < ncand; j++) {
+              const uint32_t ps = (uint32_t)__builtin_amdgcn_readlane((int)mps, j);
+              const uint32_t cid = (uint32_t)__builtin_amdgcn_readlane((int)mid, j);
+              const int cx = (int)(ps >> 16), cy = (int)(ps & 0xffffu);
+              const uint64_t *sh = a.sheds + (size_t)cid * S;
+              const int s = wave_sum_i32(
+                  eval_rows_fit(cy, roi, nwpr)
+                      ? lane_new_bits_rows<kEvalRows>(sh, cx, cy, a, gl, gw * kWave)
+                      : lane_new_bits<kEvalBatch>(sh, cx, cy, a, W, gl, gw * kWave, false));
+              if (lane == 0) atomicAdd(&list_acc[e0 + j * ngroups], s);
+            }
+          }
+          __syncthreads();
+          for (int e = tid; e < nall; e += kLoopThreads) {
+            const int i = list_i[e];
+            T[i] = list_acc[e];
+            if (e >= lcnt) {  // re-evaluated: exact again
+              U[i] = 0;
+              const uint32_t ps = P[i];
+              dirty[((int)(ps >> 16) / a.cs * Q + ((int)(ps & 0xffffu) / a.cs) % Q) / G] = 1;
+            }
+          }
+        }
+      }
       // (an intervis update reads cum | the winner's bits, the same before,
       // during and after the union: it needs no wait)
-      if (!stop && lcnt > 0 && !kIv) union_wait();
-      if (!stop && lcnt > 0 && !failed) {
+      if (!merged && !stop && lcnt > 0 && !kIv) union_wait();
+      if (!merged && !stop && lcnt > 0 && !failed) {
         // one wavefront per update candidate
         for (int e = wid; e < lcnt; e += kLoopWaves) {
           if (list_acc[e] < 0) continue;
@@ -1889,7 +1948,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
       break;
     }
     __syncthreads();
-    if (hybrid) {
+    if (hybrid && reeval_rest) {
       // re-evaluate the stale candidates whose bound reaches tau against cum
       // (complete: the union ended with a grid barrier); a group of waves
       // shares a candidate when there are few
