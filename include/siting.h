@@ -234,10 +234,10 @@ siting_status siting_job_derived(siting_job *job, siting_derived *d);
  * small pool of streams so they overlap, and every job's greedy SITE loop
  * runs in a single persistent work-queue launch whose slots of workgroups
  * take jobs, largest first, as they free up.  The slot plan is tuned by
- * measurement over the first synced runs (equal slots, then a mixed plan
- * from each job's measured loop time, then the job that finished last moved
- * onto a larger slot of its own, up to five runs), and the fastest plan
- * measured is kept.  The prep sequence is captured into a HIP graph and
+ * measurement over the first synced runs (equal slots, then the three best
+ * mixed plans of a model of each job's measured loop time, then the job
+ * that ran longest moved onto a larger slot of its own, up to seven runs),
+ * and the fastest plan measured is kept.  The prep sequence is captured into a HIP graph and
  * replayed.  Results are bit-identical to running each job
  * alone, whatever slot runs it. */
 

@@ -1058,10 +1058,11 @@ siting_status siting_batch_create(int device, int njobs, const siting_params *pa
   for (int i = 1; i < njobs; i++)
     if ((params[i].intervis ? 1 : 0) != b->iv_mode) b->iv_mode = 2;
   const int nstreams = opts && opts->streams > 0 ? opts->streams : 4;
-  // 8 workgroups per base slot measured best on the uneven 64-job batch
-  // (step ms with re-planned slots: 4: 328, 6: 313, 8: 290, 12: 295,
-  // 16: 295, 24: 307)
-  b->slot = opts && opts->slot > 0 ? opts->slot : 8;
+  // 12 workgroups per base slot measured best on the uneven 64-job batch
+  // with the current loop (SITE launch ms with tuned slots: 6: 72.4, 8: 60.2,
+  // 10: 61.0, 12: 57.6, 14: 60.7, 16: 60.6; with the loops of an earlier
+  // design, whose rounds cost several times more, 8 had been best)
+  b->slot = opts && opts->slot > 0 ? opts->slot : 12;
   if (b->slot > cus) b->slot = cus;
   b->nslots = opts && opts->nslots > 0 ? opts->nslots : cus / b->slot;
   if (b->nslots * b->slot > cus) b->nslots = cus / b->slot;
