@@ -72,15 +72,6 @@ constexpr int kMaxRoi = 10000;
 // ---------------------------------------------------------------------------
 // wave / block reductions
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    uint64_t o = __shfl_xor(v, off, kWave);
-    v = o > v ? o : v;
-  }
-  return v;
-}
-
 // Sum over the wavefront, in every lane (all lanes active), with DPP row
 // operations: quad swaps, half-row and row mirrors (every lane then holds
 // its row's sum), then the row broadcasts that fold rows 0-3 into lane 63.

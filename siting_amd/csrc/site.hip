@@ -696,18 +696,6 @@ __device__ __forceinline__ void top3_push(Top3 &t, uint64_t k, uint32_t p) {
   }
 }
 
-// Drops a lane's first key: the others move up, and the largest key the
-// lane did not keep follows them as a bound entry.
-__device__ __forceinline__ void lane_pop(Top3 &t) {
-  t.k[0] = t.k[1];
-  t.p[0] = t.p[1];
-  t.k[1] = t.k[2];
-  t.p[1] = t.p[2];
-  t.k[2] = t.over;
-  t.p[2] = t.over ? kGhost : 0;
-  t.over = 0;
-}
-
 // Best and second best (key, position) and the third best key over the
 // unused candidates of owned unit ul; one wavefront.
 struct Top2 {
