@@ -165,7 +165,7 @@ def lib() -> ctypes.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise SitingError(EDEVICE, f"{LIB_PATH} is not built "
-                              "(run __graft_entry__.build())")
+                              "(run make -C siting_amd/csrc)")
         handle = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             fn = getattr(handle, name)
