@@ -777,11 +777,11 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
   __shared__ int r_mark[kMaxRekey];
   __shared__ int bw_npair[kWave];  // earlier picks of the round within 2*roi of pick i
   __shared__ int bw_readcum;       // the walk read cum
-  // The walk's area computations are shared by the whole workgroup: the
-  // walking wavefront posts a request (candidate, picks) and takes its
-  // share, the others, waiting in a service loop, take theirs and add them
-  // to mb_acc with their arrival (arrivals << 40 | sum).  mb_seq numbers
-  // the requests (-1: the walk is over).
+  // The walk's area computations are done by the rest of the workgroup:
+  // the walking wavefront posts a request (candidate, picks) and waits, the
+  // others, waiting in a service loop, each take a share and add it to
+  // mb_acc with their arrival (arrivals << 40 | sum).  mb_seq numbers the
+  // requests (-1: the walk is over).
   __shared__ int mb_seq;
   __shared__ unsigned long long mb_acc;
   __shared__ uint32_t mb_cid, mb_cpos;
@@ -1206,10 +1206,6 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
           mb_n++;
           if (lane == 0) __hip_atomic_store(&mb_seq, mb_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-          const int own = wave_sum_i32(lane_new_bits_picks_np(
-              a.sheds + (size_t)cid * S, (int)(cpos >> 16), (int)(cpos & 0xffffu),
-              __popcll(picks), mb_ppos, mb_pid, a, lane, kLoopThreads));
           unsigned long long acc = 0;
           if (lane == 0)
             while (((acc = __hip_atomic_load(&mb_acc, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) >> 40) <
@@ -1217,7 +1213,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
               __builtin_amdgcn_s_sleep(1);
           // (lane 0's sum by readlane: a wave-uniform value, so that the
           // walk's control flow that depends on it stays scalar)
-          return own + __builtin_amdgcn_readlane((int)(uint32_t)(acc & ((1ull << 40) - 1)), 0);
+          return __builtin_amdgcn_readlane((int)(uint32_t)(acc & ((1ull << 40) - 1)), 0);
         };
         auto walk_area = [&](uint32_t cid, uint32_t cpos, uint64_t picks) -> int {
           SITE_PROF(const unsigned long long t0 = wall_clock64();)
@@ -1500,7 +1496,9 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           if (walk_help) __hip_atomic_store(&mb_seq, -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
       } else if (walk_help) {
-        // the walk's helpers: a share of each area computation it posts
+        // the walk's helpers: the 15 other wavefronts share each area
+        // computation it posts (the walking wavefront only waits; with a
+        // share of its own the loop took 1-2 % longer)
         int seen = 0;
         for (;;) {
           int q;
@@ -1511,7 +1509,7 @@ __device__ __forceinline__ void site_loop_body(const SiteArgs &a, int g, int G) 
           const uint32_t cpos = mb_cpos;
           const int part = wave_sum_i32(lane_new_bits_picks_np(
               a.sheds + (size_t)mb_cid * S, (int)(cpos >> 16), (int)(cpos & 0xffffu), mb_np,
-              mb_ppos, mb_pid, a, wid * kWave + lane, kLoopThreads));
+              mb_ppos, mb_pid, a, (wid - 1) * kWave + lane, kLoopThreads - kWave));
           if (lane == 0)
             __hip_atomic_fetch_add(&mb_acc, (1ull << 40) + (unsigned long long)part, __ATOMIC_RELEASE,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
