@@ -56,3 +56,33 @@ def test_site_knobs_match_oracle(gpu, case, knobs, intervis, monkeypatch):
     np.testing.assert_array_equal(sel, sel_r)
     assert (st.nused, st.niter, st.cumarea, st.sum_u) == \
         (st_r.nused, st_r.niter, st_r.cumarea, st_r.sum_u)
+
+
+# roi 130: a candidate's shed box has 261 rows of 5-6 terrain words, more
+# (row, word) items than the fifteen helper wavefronts' 960 lanes, so each of
+# the walk's exact computations takes them in two passes
+@pytest.fixture(scope="module")
+def case_wide(sa, orc):
+    n, roi = 700, 130
+    dem = make_dem(sa, n, 0.7, 20000, 29)
+    rng = np.random.default_rng(31)
+    obs = rng.choice(n * n, size=3000, replace=False).astype(np.int32)
+    sheds = orc.calc_sheds(n, dem, roi, 100, 100, obs)
+    refs = {iv: orc.site(n, roi, iv, obs, sheds) for iv in (0, 1)}
+    return n, roi, obs, sheds, refs
+
+
+@pytest.mark.parametrize("intervis", [0, 1])
+@pytest.mark.parametrize("knobs", [{}, {"SITING_SITE_NEAR_EVALS": "32"}],
+                         ids=lambda k: ",".join(f"{a[12:]}={b}" for a, b in k.items()) or "default")
+def test_site_wide_boxes_match_oracle(gpu, case_wide, knobs, intervis, monkeypatch):
+    n, roi, obs, sheds, refs = case_wide
+    for name, value in knobs.items():
+        monkeypatch.setenv(name, value)
+    print(f"[case] n {n} roi {roi} intervis {intervis} knobs {knobs}", flush=True)
+    sel, st = gpu.site_it(n, roi, intervis, obs.size, obs, sheds)
+    sel_r, order_r, st_r = refs[intervis]
+    assert st_r.nused > 500
+    np.testing.assert_array_equal(sel, sel_r)
+    assert (st.nused, st.niter, st.cumarea, st.sum_u) == \
+        (st_r.nused, st_r.niter, st_r.cumarea, st_r.sum_u)
